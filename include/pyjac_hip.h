@@ -1,0 +1,203 @@
+/*
+ * pyjac_hip.h -- C ABI of the MI355X (gfx950) species-rate / analytical
+ * Jacobian library.
+ *
+ * The reference exposes each generated kernel as a C++ class
+ * (kernel_utils/common/kernel.hpp.in:86-336): <Name>Kernel(problem_size,
+ * work_size), resize(), operator()(double* ...), requiredMemorySize(),
+ * speciesNames(), reactionStrings(), numSpecies(), numReactions(), order(),
+ * and Python reaches it through the Cython wrapper
+ * (pywrap/pyjacob_wrapper.pyx.in:1-92).  This header replaces that class
+ * with plain C entry points (plain pointers and sizes) that a ctypes,
+ * cgo or JNI binding can call.  Errors are returned as status codes; the
+ * reference's cassert/exit(-1) path (kernel_utils/c/error_check.cpp:9-15)
+ * has no counterpart here.
+ *
+ * Data layout is the reference's (docs/data_ordering.rst,
+ * core/array_creator.py:2039-2067):
+ *   phi   [n_states][NN] (order 'C') or [NN][n_states] (order 'F'),
+ *         phi = [T, V (CONP) | P (CONV), n_0 .. n_{Ns-2}], NN = Ns + 1
+ *   param [n_states]  P (CONP) or V (CONV)
+ *   dphi  same shape as phi
+ *   jac   full:   C: [n][NN(row)][NN(col)]   F: [NN(col)][NN(row)][n]
+ *         sparse: C: CRS values [n][nnz]     F: CCS values [nnz][n]
+ * All floating point is IEEE binary64.
+ */
+#ifndef PYJAC_HIP_H
+#define PYJAC_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PYJAC_MECH_ABI_VERSION 1
+
+/* status codes */
+#define PYJAC_OK 0
+#define PYJAC_ERR_ARG 1          /* bad argument (null pointer, bad size) */
+#define PYJAC_ERR_HIP 2          /* a HIP runtime call failed */
+#define PYJAC_ERR_ALLOC 3        /* device or host allocation failed */
+#define PYJAC_ERR_UNSUPPORTED 4  /* option combination not supported */
+#define PYJAC_ERR_STATE 5        /* handle not ready (e.g. not resized) */
+
+/* Arrhenius forms of simple_form / fall_form (rate_subs.py:100-124) */
+#define PYJAC_ARR_A 0          /* kf = A */
+#define PYJAC_ARR_AT_B 1       /* kf = A * T^b, integer b */
+#define PYJAC_ARR_FULL 2       /* kf = exp(logA + b logT - Ta/T) */
+#define PYJAC_ARR_NO_TA 3      /* kf = exp(logA + b logT) */
+#define PYJAC_ARR_NO_B 4       /* kf = exp(logA - Ta/T) */
+
+/* Jacobian formats / types (core/enum_types.py:85-115) */
+#define PYJAC_JAC_FULL 0
+#define PYJAC_JAC_SPARSE 1
+#define PYJAC_JAC_EXACT 0
+#define PYJAC_JAC_APPROXIMATE 1
+
+/*
+ * Mechanism tables.  Same content as the dict built by the reference's
+ * assign_rates (core/rate_subs.py:74-537) and determine_jac_inds
+ * (core/create_jacobian.py:68-282); the caller owns every array.
+ * Indices are int32 (core/array_creator.py:554).
+ */
+typedef struct pyjac_mech_tables {
+    int32_t abi_version;
+    int32_t n_species;        /* Ns, bath gas last */
+    int32_t n_reactions;      /* Nr */
+    int32_t n_rxn_spec;       /* number of (reaction, species) pairs */
+    int32_t n_simple, n_rev, n_thd, n_thd_spec, n_fall, n_troe, n_sri;
+    int32_t n_plog, plog_max_p;
+    int32_t n_cheb, cheb_max_t, cheb_max_p;
+    int32_t n_rxn_has_ns;
+    int32_t jac_nnz;          /* non-zeros of the exact/approximate pattern */
+
+    /* reaction -> participating species (sorted), rate_subs.py:126-160 */
+    const int32_t *rxn_off;   /* [Nr+1] */
+    const int32_t *rxn_spec;  /* [n_rxn_spec] */
+    const int32_t *nu_prod;   /* [n_rxn_spec] */
+    const int32_t *nu_reac;   /* [n_rxn_spec] */
+    const int32_t *nu_sum;    /* [Nr] net order change, for Kc */
+
+    /* per-reaction indices into the sub-tables, -1 when absent */
+    const int32_t *rev_mask;     /* [Nr] -> reverse index   */
+    const int32_t *thd_mask;     /* [Nr] -> third-body index */
+    const int32_t *simple_mask;  /* [Nr] -> simple index     */
+    const int32_t *plog_mask;    /* [Nr] -> PLOG index       */
+    const int32_t *cheb_mask;    /* [Nr] -> Chebyshev index  */
+    const int32_t *fall_mask;    /* [Nr] -> falloff index    */
+
+    /* simple Arrhenius [n_simple]; A holds log(A) for the exp forms */
+    const int32_t *simple_form;
+    const double *simple_A, *simple_b, *simple_Ta;
+
+    /* falloff / chemically activated [n_fall] (rate_subs.py:326-385) */
+    const int32_t *fall_map;     /* reaction index */
+    const int32_t *fall_type;    /* 0 falloff (k0 from LOW), 1 chem. act. */
+    const int32_t *fall_form;    /* Arrhenius form of the LOW/HIGH rate */
+    const int32_t *fall_blend;   /* 1 Lindemann, 2 Troe, 3 SRI */
+    const int32_t *fall_param;   /* index into troe_* or sri_* */
+    const int32_t *fall_to_thd;  /* third-body index of the reaction */
+    const double *fall_A, *fall_b, *fall_Ta;
+    const double *troe_a, *troe_T1inv, *troe_T2, *troe_T3inv;  /* [n_troe] */
+    const double *sri_a, *sri_b, *sri_c, *sri_d, *sri_e;        /* [n_sri] */
+
+    /* third bodies [n_thd] (rate_subs.py:387-426) */
+    const int32_t *thd_map;   /* reaction index */
+    const int32_t *thd_type;  /* 1 mix, 2 single species, 3 unity */
+    const int32_t *thd_off;   /* [n_thd+1] into thd_spec / thd_eff */
+    const int32_t *thd_spec;  /* [n_thd_spec] */
+    const double *thd_eff;    /* [n_thd_spec] */
+
+    /* PLOG [n_plog]; params [4][n_plog][plog_max_p] = log P, log A, b, Ta */
+    const int32_t *plog_map, *plog_np;
+    const double *plog_params;
+
+    /* Chebyshev [n_cheb]; params [n_cheb][cheb_max_t][cheb_max_p],
+       plim [n_cheb][2] = log P limits, tlim [n_cheb][2] = 1/T limits */
+    const int32_t *cheb_map, *cheb_nt, *cheb_np;
+    const double *cheb_params, *cheb_plim, *cheb_tlim;
+
+    /* NASA-7 [Ns][7] low / high coefficients, T_mid [Ns],
+       mw_post [Ns-1] = mw_k / mw_ns */
+    const double *a_lo, *a_hi, *T_mid, *mw_post;
+
+    /* reactions in which the bath gas takes part [n_rxn_has_ns] */
+    const int32_t *rxn_has_ns;
+
+    /* Jacobian pattern, CCS (F order) and CRS (C order) */
+    const int32_t *ccs_col_ptr;  /* [NN+1] */
+    const int32_t *ccs_row_ind;  /* [jac_nnz] */
+    const int32_t *crs_row_ptr;  /* [NN+1] */
+    const int32_t *crs_col_ind;  /* [jac_nnz] */
+} pyjac_mech_tables;
+
+typedef struct pyjac_handle pyjac_handle;
+
+/* Last error message of the calling thread ("" when none). */
+const char *pyjac_hip_last_error(void);
+
+/*
+ * Create a handle bound to one mechanism and option set; uploads the tables
+ * to the current device.  Replaces the generated <Name>Kernel constructor
+ * (kernel.cpp.in:40-76).
+ *   conp        1 constant pressure, 0 constant volume (utils.py:972)
+ *   order       'C' or 'F' (utils.py:944)
+ *   jac_format  PYJAC_JAC_FULL or PYJAC_JAC_SPARSE (utils.py:1014)
+ *   jac_type    PYJAC_JAC_EXACT or PYJAC_JAC_APPROXIMATE (utils.py:999)
+ */
+int pyjac_hip_create(const pyjac_mech_tables *mech, int conp, char order,
+                     int jac_format, int jac_type, pyjac_handle **out);
+
+/* Release all device memory of the handle (~Kernel, kernel.cpp.in:300). */
+int pyjac_hip_destroy(pyjac_handle *h);
+
+/*
+ * Size the internal staging buffers for host-pointer calls of up to
+ * problem_size states (Kernel::resize, kernel.cpp.in:60-76).  work_size is
+ * accepted for signature compatibility and sets the number of states moved
+ * per staging chunk when non-zero.
+ */
+int pyjac_hip_resize(pyjac_handle *h, size_t problem_size, size_t work_size);
+
+/* Host-pointer entry points: copy in, compute, copy out, synchronous.
+   Replace <Name>Kernel::operator() (kernel.cpp.in:572-640). */
+int pyjac_hip_species_rates(pyjac_handle *h, const double *param,
+                            const double *phi, double *dphi);
+int pyjac_hip_jacobian(pyjac_handle *h, const double *param,
+                       const double *phi, double *jac);
+/* chem_utils kernel (rate_subs.py:3391-3452): spec_energy = h (CONP) or
+   u (CONV), spec_heat = cp or cv, b = S/R - H/RT - ln T;  arrays shaped
+   like phi but with Ns columns. */
+int pyjac_hip_chem_utils(pyjac_handle *h, const double *phi,
+                         double *spec_energy, double *spec_heat, double *b);
+
+/* Device-pointer entry points: n_states states already resident in device
+   memory in the handle's data order, launched on `stream` (a hipStream_t,
+   NULL for the default stream), asynchronous. */
+int pyjac_hip_species_rates_dev(pyjac_handle *h, size_t n_states,
+                                const double *d_param, const double *d_phi,
+                                double *d_dphi, void *stream);
+int pyjac_hip_jacobian_dev(pyjac_handle *h, size_t n_states,
+                           const double *d_param, const double *d_phi,
+                           double *d_jac, void *stream);
+
+/* Introspection (Kernel::numSpecies etc., kernel.hpp.in:180-268). */
+int pyjac_hip_num_species(const pyjac_handle *h);
+int pyjac_hip_num_reactions(const pyjac_handle *h);
+int pyjac_hip_jacobian_size(const pyjac_handle *h); /* NN*NN or nnz */
+/* Device workspace bytes per state of the Jacobian / rates kernels
+   (Kernel::requiredMemorySize, kernel.hpp.in:220). */
+size_t pyjac_hip_required_memory(const pyjac_handle *h);
+
+/* Select the kernel variant: 0 automatic (tuned), else a variant id
+   returned by pyjac_hip_variant_count / used by the autotuner. */
+int pyjac_hip_set_variant(pyjac_handle *h, int kernel, int variant);
+int pyjac_hip_get_variant(const pyjac_handle *h, int kernel);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PYJAC_HIP_H */

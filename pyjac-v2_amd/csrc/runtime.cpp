@@ -1,0 +1,424 @@
+// runtime.cpp -- C ABI of libpyjac_hip.so (declared in include/pyjac_hip.h).
+//
+// Replaces the reference's generated C++ Kernel class
+// (kernel_utils/common/kernel.hpp.in:86-336, kernel.cpp.in:40-640): a handle
+// owns the device copy of the mechanism tables, the per-wave workspace of
+// the persistent kernels and the staging buffers of the host-pointer entry
+// points.  Every entry point returns a status code; the message of the last
+// failure of the calling thread is available from pyjac_hip_last_error().
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/pyjac_hip.h"
+#include "device_mech.h"
+#include "kernels.h"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                   \
+    do {                                                                                \
+        hipError_t e_ = (expr);                                                         \
+        if (e_ != hipSuccess)                                                           \
+            return fail(PYJAC_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+// Packs host arrays into one blob, remembering where each landed.
+struct Packer {
+    std::vector<char> bytes;
+    template <class T>
+    size_t add(const T* p, size_t n) {
+        size_t off = (bytes.size() + 255) & ~size_t(255);
+        bytes.resize(off + std::max<size_t>(n, 1) * sizeof(T), 0);
+        if (n) std::memcpy(bytes.data() + off, p, n * sizeof(T));
+        return off;
+    }
+};
+
+}  // namespace
+
+struct pyjac_handle {
+    int conp = 1;
+    char order = 'C';
+    int jac_format = PYJAC_JAC_SPARSE;
+    int jac_type = PYJAC_JAC_EXACT;
+    int device = 0;
+    int n_cu = 256;
+    DevMech m{};
+    char* d_blob = nullptr;
+    // workspace of the persistent kernels
+    double* d_ws = nullptr;
+    size_t ws_bytes = 0;
+    // staging for host-pointer calls
+    size_t problem_size = 0, chunk = 0;
+    double *d_param = nullptr, *d_phi = nullptr, *d_out = nullptr, *d_out2 = nullptr,
+           *d_out3 = nullptr;
+    size_t staged_states = 0;
+    int grid_rates = 0, grid_jac = 0, block = 256;
+    int variant[4] = {0, 0, 0, 0};
+    hipStream_t stream = nullptr;
+};
+
+namespace {
+
+int nvalues(const pyjac_handle* h) {
+    int NN = h->m.NN;
+    return h->jac_format == PYJAC_JAC_FULL ? NN * NN : h->m.jac_nnz;
+}
+
+// grid for a persistent kernel: enough waves to fill the device, capped
+// so that the workspace stays within the budget
+int pick_grid(const pyjac_handle* h, size_t per_state_doubles, int variant, long n) {
+    int waves_per_cu = 8;
+    if (variant > 0) waves_per_cu = variant;  // tuned: waves per CU
+    const char* env = getenv("PYJAC_HIP_WAVES_PER_CU");
+    if (env && atoi(env) > 0) waves_per_cu = atoi(env);
+    long blocks = (long)h->n_cu * waves_per_cu / (h->block / 64);
+    size_t per_block = per_state_doubles * (size_t)h->block * sizeof(double);
+    size_t budget = (size_t)24 << 30;  // 24 GiB of workspace at most
+    long cap = (long)std::max<size_t>(1, budget / std::max<size_t>(per_block, 1));
+    blocks = std::min(blocks, cap);
+    long need = (n + h->block - 1) / h->block;
+    blocks = std::max(1L, std::min(blocks, need));
+    return (int)blocks;
+}
+
+int ensure_ws(pyjac_handle* h, size_t bytes) {
+    if (bytes <= h->ws_bytes) return PYJAC_OK;
+    if (h->d_ws) (void)hipFree(h->d_ws);
+    h->d_ws = nullptr;
+    h->ws_bytes = 0;
+    HIP_TRY(hipMalloc(&h->d_ws, bytes));
+    h->ws_bytes = bytes;
+    return PYJAC_OK;
+}
+
+int run_rates(pyjac_handle* h, long n, const double* param, const double* phi, double* dphi,
+              hipStream_t s) {
+    if (n <= 0) return PYJAC_OK;
+    size_t per = pjk_host::ws_doubles_per_state(h->m, false);
+    int grid = pick_grid(h, per, h->variant[0], n);
+    int rc = ensure_ws(h, per * (size_t)grid * h->block * sizeof(double));
+    if (rc) return rc;
+    HIP_TRY(pjk_host::launch_species_rates(h->m, h->conp, h->order, n, param, phi, dphi,
+                                           h->d_ws, grid, h->block, s));
+    return PYJAC_OK;
+}
+
+int run_jac(pyjac_handle* h, long n, const double* param, const double* phi, double* jac,
+            hipStream_t s) {
+    if (n <= 0) return PYJAC_OK;
+    size_t per = pjk_host::ws_doubles_per_state(h->m, true);
+    int grid = pick_grid(h, per, h->variant[1], n);
+    int rc = ensure_ws(h, per * (size_t)grid * h->block * sizeof(double));
+    if (rc) return rc;
+    HIP_TRY(pjk_host::launch_jacobian(h->m, h->conp, h->order, n, param, phi, jac, h->d_ws, grid,
+                                      h->block, s));
+    return PYJAC_OK;
+}
+
+// host <-> device copy of `cnt` states starting at `off` of an array with
+// `width` values per state, in the handle's data order
+int copy_states(pyjac_handle* h, double* dst, const double* src, size_t total, size_t off,
+                size_t cnt, size_t width, bool to_device, hipStream_t s) {
+    hipMemcpyKind kind = to_device ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost;
+    if (h->order == 'C' || width == 1) {
+        if (to_device)
+            HIP_TRY(hipMemcpyAsync(dst, src + off * width, cnt * width * sizeof(double), kind, s));
+        else
+            HIP_TRY(hipMemcpyAsync(dst + off * width, src, cnt * width * sizeof(double), kind, s));
+    } else {
+        // F order: `width` rows of `total` states on the host, rows of `cnt`
+        // states on the device (kernel_utils/c/memcpy_2d.hpp)
+        if (to_device)
+            HIP_TRY(hipMemcpy2DAsync(dst, cnt * sizeof(double), src + off, total * sizeof(double),
+                                     cnt * sizeof(double), width, kind, s));
+        else
+            HIP_TRY(hipMemcpy2DAsync(dst + off, total * sizeof(double), src, cnt * sizeof(double),
+                                     cnt * sizeof(double), width, kind, s));
+    }
+    return PYJAC_OK;
+}
+
+int ensure_staging(pyjac_handle* h) {
+    if (!h->problem_size) return fail(PYJAC_ERR_STATE, "handle not resized (problem_size == 0)");
+    size_t cnt = h->chunk ? std::min(h->chunk, h->problem_size) : h->problem_size;
+    if (cnt <= h->staged_states && h->d_param) return PYJAC_OK;
+    (void)hipFree(h->d_param); (void)hipFree(h->d_phi); (void)hipFree(h->d_out); (void)hipFree(h->d_out2);
+    (void)hipFree(h->d_out3);
+    h->d_param = h->d_phi = h->d_out = h->d_out2 = h->d_out3 = nullptr;
+    size_t NN = h->m.NN;
+    size_t outw = std::max<size_t>(NN, (size_t)nvalues(h));
+    HIP_TRY(hipMalloc(&h->d_param, cnt * sizeof(double)));
+    HIP_TRY(hipMalloc(&h->d_phi, cnt * NN * sizeof(double)));
+    HIP_TRY(hipMalloc(&h->d_out, cnt * outw * sizeof(double)));
+    HIP_TRY(hipMalloc(&h->d_out2, cnt * NN * sizeof(double)));
+    HIP_TRY(hipMalloc(&h->d_out3, cnt * NN * sizeof(double)));
+    h->staged_states = cnt;
+    return PYJAC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* pyjac_hip_last_error(void) { return g_last_error.c_str(); }
+
+int pyjac_hip_create(const pyjac_mech_tables* mt, int conp, char order, int jac_format,
+                     int jac_type, pyjac_handle** out) {
+    if (!mt || !out) return fail(PYJAC_ERR_ARG, "null argument");
+    *out = nullptr;
+    if (mt->abi_version != PYJAC_MECH_ABI_VERSION)
+        return fail(PYJAC_ERR_ARG, "mechanism table ABI %d, library expects %d", mt->abi_version,
+                    PYJAC_MECH_ABI_VERSION);
+    if (order != 'C' && order != 'F') return fail(PYJAC_ERR_ARG, "order must be 'C' or 'F'");
+    if (jac_format != PYJAC_JAC_FULL && jac_format != PYJAC_JAC_SPARSE)
+        return fail(PYJAC_ERR_ARG, "bad jac_format %d", jac_format);
+    if (jac_type != PYJAC_JAC_EXACT && jac_type != PYJAC_JAC_APPROXIMATE)
+        return fail(PYJAC_ERR_UNSUPPORTED, "jac_type %d not supported by the HIP backend",
+                    jac_type);
+    const int Ns = mt->n_species, Nr = mt->n_reactions, NN = Ns + 1;
+    if (Ns < 2 || Nr < 1) return fail(PYJAC_ERR_ARG, "empty mechanism");
+    if (mt->n_cheb && (mt->cheb_max_p > 16 || mt->cheb_max_t > 16))
+        return fail(PYJAC_ERR_UNSUPPORTED, "Chebyshev degree above 16");
+
+    pyjac_handle* h = new pyjac_handle();
+    h->conp = conp ? 1 : 0;
+    h->order = order;
+    h->jac_format = jac_format;
+    h->jac_type = jac_type;
+    (void)hipGetDevice(&h->device);
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, h->device) == hipSuccess && prop.multiProcessorCount > 0)
+        h->n_cu = prop.multiProcessorCount;
+
+    // derived tables
+    std::vector<int> thd_only, fall_order, jac_pos;
+    for (int t = 0; t < mt->n_thd; ++t)
+        if (mt->fall_mask[mt->thd_map[t]] < 0) thd_only.push_back(t);
+    for (int blend : {1, 3, 2})
+        for (int f = 0; f < mt->n_fall; ++f)
+            if (mt->fall_blend[f] == blend) fall_order.push_back(f);
+    if (jac_format == PYJAC_JAC_FULL) {
+        for (int a = 0; a < NN; ++a)
+            for (int b = 0; b < NN; ++b)
+                jac_pos.push_back(order == 'C' ? a * NN + b : b * NN + a);
+    } else if (order == 'F') {
+        for (int col = 0; col < NN; ++col)
+            for (int q = mt->ccs_col_ptr[col]; q < mt->ccs_col_ptr[col + 1]; ++q)
+                jac_pos.push_back(mt->ccs_row_ind[q] * NN + col);
+    } else {
+        for (int row = 0; row < NN; ++row)
+            for (int q = mt->crs_row_ptr[row]; q < mt->crs_row_ptr[row + 1]; ++q)
+                jac_pos.push_back(row * NN + mt->crs_col_ind[q]);
+    }
+
+    Packer P;
+    DevMech& m = h->m;
+    m.Ns = Ns; m.Nr = Nr; m.NN = NN; m.ns = Ns - 1;
+    m.n_rev = mt->n_rev; m.n_thd = mt->n_thd; m.n_fall = mt->n_fall;
+    m.n_plog = mt->n_plog; m.plog_max_p = mt->plog_max_p;
+    m.n_cheb = mt->n_cheb; m.cheb_max_t = mt->cheb_max_t; m.cheb_max_p = mt->cheb_max_p;
+    m.n_rxn_has_ns = mt->n_rxn_has_ns;
+    m.jac_nnz = (int)jac_pos.size();
+    m.n_thd_only = (int)thd_only.size();
+    const int nrs = mt->n_rxn_spec, nts = mt->n_thd_spec;
+    const int nf = mt->n_fall, nt = mt->n_thd;
+    struct Slot { const void** dst; size_t off; };
+    std::vector<Slot> slots;
+#define PACK(field, src, count) \
+    slots.push_back({(const void**)&m.field, P.add(src, (size_t)(count))})
+    PACK(rxn_off, mt->rxn_off, Nr + 1);
+    PACK(rxn_spec, mt->rxn_spec, nrs);
+    PACK(nu_prod, mt->nu_prod, nrs);
+    PACK(nu_reac, mt->nu_reac, nrs);
+    PACK(nu_sum, mt->nu_sum, Nr);
+    PACK(rev_mask, mt->rev_mask, Nr);
+    PACK(thd_mask, mt->thd_mask, Nr);
+    PACK(simple_mask, mt->simple_mask, Nr);
+    PACK(plog_mask, mt->plog_mask, Nr);
+    PACK(cheb_mask, mt->cheb_mask, Nr);
+    PACK(fall_mask, mt->fall_mask, Nr);
+    PACK(simple_form, mt->simple_form, mt->n_simple);
+    PACK(simple_A, mt->simple_A, mt->n_simple);
+    PACK(simple_b, mt->simple_b, mt->n_simple);
+    PACK(simple_Ta, mt->simple_Ta, mt->n_simple);
+    PACK(fall_map, mt->fall_map, nf);
+    PACK(fall_type, mt->fall_type, nf);
+    PACK(fall_form, mt->fall_form, nf);
+    PACK(fall_blend, mt->fall_blend, nf);
+    PACK(fall_param, mt->fall_param, nf);
+    PACK(fall_to_thd, mt->fall_to_thd, nf);
+    PACK(fall_A, mt->fall_A, nf);
+    PACK(fall_b, mt->fall_b, nf);
+    PACK(fall_Ta, mt->fall_Ta, nf);
+    PACK(troe_a, mt->troe_a, mt->n_troe);
+    PACK(troe_T1inv, mt->troe_T1inv, mt->n_troe);
+    PACK(troe_T2, mt->troe_T2, mt->n_troe);
+    PACK(troe_T3inv, mt->troe_T3inv, mt->n_troe);
+    PACK(sri_a, mt->sri_a, mt->n_sri);
+    PACK(sri_b, mt->sri_b, mt->n_sri);
+    PACK(sri_c, mt->sri_c, mt->n_sri);
+    PACK(sri_d, mt->sri_d, mt->n_sri);
+    PACK(sri_e, mt->sri_e, mt->n_sri);
+    PACK(thd_map, mt->thd_map, nt);
+    PACK(thd_type, mt->thd_type, nt);
+    PACK(thd_off, mt->thd_off, nt + 1);
+    PACK(thd_spec, mt->thd_spec, nts);
+    PACK(thd_eff, mt->thd_eff, nts);
+    PACK(plog_map, mt->plog_map, mt->n_plog);
+    PACK(plog_np, mt->plog_np, mt->n_plog);
+    PACK(plog_params, mt->plog_params, (size_t)4 * mt->n_plog * mt->plog_max_p);
+    PACK(cheb_map, mt->cheb_map, mt->n_cheb);
+    PACK(cheb_nt, mt->cheb_nt, mt->n_cheb);
+    PACK(cheb_np, mt->cheb_np, mt->n_cheb);
+    PACK(cheb_params, mt->cheb_params, (size_t)mt->n_cheb * mt->cheb_max_t * mt->cheb_max_p);
+    PACK(cheb_plim, mt->cheb_plim, 2 * mt->n_cheb);
+    PACK(cheb_tlim, mt->cheb_tlim, 2 * mt->n_cheb);
+    PACK(a_lo, mt->a_lo, 7 * Ns);
+    PACK(a_hi, mt->a_hi, 7 * Ns);
+    PACK(T_mid, mt->T_mid, Ns);
+    PACK(mw_post, mt->mw_post, Ns - 1);
+    PACK(rxn_has_ns, mt->rxn_has_ns, mt->n_rxn_has_ns);
+    PACK(thd_only, thd_only.data(), thd_only.size());
+    PACK(fall_order, fall_order.data(), fall_order.size());
+    PACK(jac_pos, jac_pos.data(), jac_pos.size());
+#undef PACK
+    hipError_t e = hipMalloc(&h->d_blob, P.bytes.size());
+    if (e == hipSuccess)
+        e = hipMemcpy(h->d_blob, P.bytes.data(), P.bytes.size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        (void)hipFree(h->d_blob);
+        delete h;
+        return fail(PYJAC_ERR_HIP, "table upload failed: %s", hipGetErrorString(e));
+    }
+    for (auto& s : slots) *s.dst = h->d_blob + s.off;
+    *out = h;
+    return PYJAC_OK;
+}
+
+int pyjac_hip_destroy(pyjac_handle* h) {
+    if (!h) return PYJAC_OK;
+    (void)hipFree(h->d_blob);
+    (void)hipFree(h->d_ws);
+    (void)hipFree(h->d_param);
+    (void)hipFree(h->d_phi);
+    (void)hipFree(h->d_out);
+    (void)hipFree(h->d_out2);
+    (void)hipFree(h->d_out3);
+    delete h;
+    return PYJAC_OK;
+}
+
+int pyjac_hip_resize(pyjac_handle* h, size_t problem_size, size_t work_size) {
+    if (!h) return fail(PYJAC_ERR_ARG, "null handle");
+    h->problem_size = problem_size;
+    h->chunk = work_size;
+    return PYJAC_OK;
+}
+
+int pyjac_hip_species_rates_dev(pyjac_handle* h, size_t n, const double* param, const double* phi,
+                                double* dphi, void* stream) {
+    if (!h || (n && (!param || !phi || !dphi))) return fail(PYJAC_ERR_ARG, "null argument");
+    return run_rates(h, (long)n, param, phi, dphi, (hipStream_t)stream);
+}
+
+int pyjac_hip_jacobian_dev(pyjac_handle* h, size_t n, const double* param, const double* phi,
+                           double* jac, void* stream) {
+    if (!h || (n && (!param || !phi || !jac))) return fail(PYJAC_ERR_ARG, "null argument");
+    return run_jac(h, (long)n, param, phi, jac, (hipStream_t)stream);
+}
+
+int pyjac_hip_species_rates(pyjac_handle* h, const double* param, const double* phi,
+                            double* dphi) {
+    if (!h || !param || !phi || !dphi) return fail(PYJAC_ERR_ARG, "null argument");
+    int rc = ensure_staging(h);
+    if (rc) return rc;
+    const size_t N = h->problem_size, NN = h->m.NN, per = h->staged_states;
+    for (size_t off = 0; off < N; off += per) {
+        size_t cnt = std::min(per, N - off);
+        if ((rc = copy_states(h, h->d_param, param, N, off, cnt, 1, true, 0))) return rc;
+        if ((rc = copy_states(h, h->d_phi, phi, N, off, cnt, NN, true, 0))) return rc;
+        if ((rc = run_rates(h, (long)cnt, h->d_param, h->d_phi, h->d_out, 0))) return rc;
+        if ((rc = copy_states(h, dphi, h->d_out, N, off, cnt, NN, false, 0))) return rc;
+    }
+    HIP_TRY(hipStreamSynchronize(0));
+    return PYJAC_OK;
+}
+
+int pyjac_hip_jacobian(pyjac_handle* h, const double* param, const double* phi, double* jac) {
+    if (!h || !param || !phi || !jac) return fail(PYJAC_ERR_ARG, "null argument");
+    int rc = ensure_staging(h);
+    if (rc) return rc;
+    const size_t N = h->problem_size, NN = h->m.NN, per = h->staged_states;
+    const size_t nv = nvalues(h);
+    for (size_t off = 0; off < N; off += per) {
+        size_t cnt = std::min(per, N - off);
+        if ((rc = copy_states(h, h->d_param, param, N, off, cnt, 1, true, 0))) return rc;
+        if ((rc = copy_states(h, h->d_phi, phi, N, off, cnt, NN, true, 0))) return rc;
+        if ((rc = run_jac(h, (long)cnt, h->d_param, h->d_phi, h->d_out, 0))) return rc;
+        if ((rc = copy_states(h, jac, h->d_out, N, off, cnt, nv, false, 0))) return rc;
+    }
+    HIP_TRY(hipStreamSynchronize(0));
+    return PYJAC_OK;
+}
+
+int pyjac_hip_chem_utils(pyjac_handle* h, const double* phi, double* energy, double* heat,
+                         double* b) {
+    if (!h || !phi || !energy || !heat || !b) return fail(PYJAC_ERR_ARG, "null argument");
+    int rc = ensure_staging(h);
+    if (rc) return rc;
+    const size_t N = h->problem_size, NN = h->m.NN, Ns = h->m.Ns, per = h->staged_states;
+    for (size_t off = 0; off < N; off += per) {
+        size_t cnt = std::min(per, N - off);
+        if ((rc = copy_states(h, h->d_phi, phi, N, off, cnt, NN, true, 0))) return rc;
+        HIP_TRY(pjk_host::launch_chem_utils(h->m, h->conp, h->order, (long)cnt, h->d_phi,
+                                            h->d_out, h->d_out2, h->d_out3, 0));
+        if ((rc = copy_states(h, energy, h->d_out, N, off, cnt, Ns, false, 0))) return rc;
+        if ((rc = copy_states(h, heat, h->d_out2, N, off, cnt, Ns, false, 0))) return rc;
+        if ((rc = copy_states(h, b, h->d_out3, N, off, cnt, Ns, false, 0))) return rc;
+    }
+    HIP_TRY(hipStreamSynchronize(0));
+    return PYJAC_OK;
+}
+
+int pyjac_hip_num_species(const pyjac_handle* h) { return h ? h->m.Ns : -1; }
+int pyjac_hip_num_reactions(const pyjac_handle* h) { return h ? h->m.Nr : -1; }
+int pyjac_hip_jacobian_size(const pyjac_handle* h) { return h ? nvalues(h) : -1; }
+
+size_t pyjac_hip_required_memory(const pyjac_handle* h) {
+    if (!h) return 0;
+    return pjk_host::ws_doubles_per_state(h->m, true) * sizeof(double);
+}
+
+int pyjac_hip_set_variant(pyjac_handle* h, int kernel, int variant) {
+    if (!h || kernel < 0 || kernel > 3 || variant < 0) return fail(PYJAC_ERR_ARG, "bad variant");
+    h->variant[kernel] = variant;
+    return PYJAC_OK;
+}
+
+int pyjac_hip_get_variant(const pyjac_handle* h, int kernel) {
+    if (!h || kernel < 0 || kernel > 3) return -1;
+    return h->variant[kernel];
+}
+
+}  // extern "C"

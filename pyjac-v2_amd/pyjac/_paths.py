@@ -1,0 +1,18 @@
+"""Locations of the bundled mechanism inputs."""
+import os
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(os.path.dirname(PKG))
+MECH_DIR = os.path.join(REPO, 'tests', 'golden', 'mechanisms')
+
+MECHANISMS = {
+    'h2o2': ('h2o2.inp', None),
+    'gri30': ('grimech30.inp', 'thermo30.dat'),
+    'test': ('test.inp', None),
+}
+
+
+def mechanism_files(name):
+    mech, therm = MECHANISMS[name]
+    return (os.path.join(MECH_DIR, mech),
+            os.path.join(MECH_DIR, therm) if therm else None)

@@ -1,0 +1,50 @@
+"""Seeded thermochemical state samplers.
+
+A state is ``phi = [T, V (CONP) | P (CONV), n_0 .. n_{Ns-2}]`` with the
+parameter array holding ``P`` (CONP) or ``V`` (CONV), reference
+core/array_creator.py:2051-2067.  Compositions are drawn as mass fractions
+from a symmetric Dirichlet distribution, converted to mole fractions with the
+molecular weights and to moles with the ideal-gas law at the sampled T, P and
+V.
+"""
+
+import numpy as np
+
+from .chem_model import RU, PA
+
+
+def sample_states(mech, n, seed=0, T_range=(800.0, 2400.0),
+                  P_range_atm=(1.0, 50.0), alpha=0.5, conp=True, V=1.0,
+                  order='C'):
+    """Return ``(param, phi)``; ``phi`` is ``[n, NN]`` for order 'C' and
+    ``[NN, n]`` for order 'F'."""
+    rng = np.random.default_rng(seed)
+    Ns = mech.Ns
+    T = rng.uniform(T_range[0], T_range[1], size=n)
+    logP = rng.uniform(np.log(P_range_atm[0]), np.log(P_range_atm[1]), size=n)
+    P = np.exp(logP) * PA
+    Y = rng.dirichlet(np.full(Ns, alpha), size=n)
+    X = Y / mech.mw[None, :]
+    X /= X.sum(axis=1, keepdims=True)
+    ntot = P * V / (RU * T)
+    moles = X * ntot[:, None]
+    phi = np.empty((n, Ns + 1))
+    phi[:, 0] = T
+    phi[:, 1] = V if conp else P
+    phi[:, 2:] = moles[:, :Ns - 1]
+    param = P.copy() if conp else np.full(n, V, dtype=np.float64)
+    if order == 'F':
+        phi = np.ascontiguousarray(phi.T)
+    return param, phi
+
+
+# The workloads of the benchmark configurations (mechanism, CONP/CONV, data
+# order, Jacobian format, number of states, temperature range, Dirichlet
+# concentration).
+WORKLOADS = {
+    'h2o2_conp_c_65536': dict(mech='h2o2', conp=True, order='C', jac_format=0,
+                              n=65536, T_range=(800.0, 2400.0), alpha=0.5),
+    'gri30_conv_f_1048576': dict(mech='gri30', conp=False, order='F',
+                                 jac_format=1, n=1048576,
+                                 T_range=(1000.0, 2800.0), alpha=0.3),
+}

@@ -1,0 +1,67 @@
+"""Build of the HIP library for gfx950.
+
+Replaces the reference's ``libgen`` (``pyjac/libgen/libgen.py:24-283``, which
+compiles the generated C/OpenCL with ``g++ -O3 -fopenmp``): here one
+mechanism-independent shared library, ``libpyjac_hip.so``, is compiled once
+with ``hipcc --offload-arch=gfx950`` and every mechanism is handed to it as
+tables at run time.
+"""
+
+import os
+import shutil
+import subprocess
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ROOT = os.path.dirname(PKG_DIR)
+CSRC = os.path.join(ROOT, 'csrc')
+INCLUDE = os.path.join(os.path.dirname(ROOT), 'include')
+LIB_NAME = 'libpyjac_hip.so'
+LIB_PATH = os.path.join(PKG_DIR, LIB_NAME)
+
+SOURCES = ['pyjac_kernels.hip', 'surrogate_kernels.hip', 'runtime.cpp',
+           'surrogate_runtime.cpp']
+HEADERS = ['device_mech.h', 'kernels.h']
+
+# every HIP translation unit is compiled with packed FP32 disabled
+HIPCC_FLAGS = ['--offload-arch=gfx950', '-O3', '-std=c++17', '-fPIC',
+               '-shared', '-Xclang', '-target-feature', '-Xclang',
+               '-packed-fp32-ops']
+
+
+def hipcc():
+    path = shutil.which('hipcc') or '/opt/rocm/bin/hipcc'
+    if not os.path.exists(path):
+        raise RuntimeError('hipcc not found')
+    return path
+
+
+def build_library(out_path=LIB_PATH, extra_flags=(), verbose=False):
+    """Compile ``libpyjac_hip.so`` into ``out_path`` (always recompiles)."""
+    srcs = [os.path.join(CSRC, s) for s in SOURCES
+            if os.path.exists(os.path.join(CSRC, s))]
+    tmp = out_path + '.tmp'
+    cmd = [hipcc()] + HIPCC_FLAGS + list(extra_flags) + \
+        ['-I', INCLUDE, '-o', tmp] + srcs
+    res = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                         text=True)
+    if res.returncode != 0:
+        raise RuntimeError('hipcc failed:\n' + res.stdout)
+    if verbose:
+        print(res.stdout)
+    os.replace(tmp, out_path)
+    return out_path
+
+
+def generate_library(lang='hip', source_dir=None, out_dir=None, **kwargs):
+    """Counterpart of the reference's ``libgen.generate_library``
+    (libgen.py:283): builds the HIP library and returns its path.  With
+    ``out_dir`` the library is also copied there."""
+    if lang != 'hip':
+        raise NotImplementedError('this build provides the hip target only')
+    path = build_library()
+    if out_dir:
+        os.makedirs(out_dir, exist_ok=True)
+        dst = os.path.join(out_dir, LIB_NAME)
+        shutil.copy2(path, dst)
+        return dst
+    return path

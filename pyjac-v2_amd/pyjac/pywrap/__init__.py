@@ -1,0 +1,247 @@
+"""Python bindings of the HIP library (ctypes over the C ABI).
+
+Mirrors the reference's Cython wrapper (``pywrap/pyjacob_wrapper.pyx.in``):
+``Py<Name>Kernel(problem_size, work_size, do_not_compile=False)`` with
+``species_names()``, ``reaction_strings()``, ``num_species()``,
+``num_reactions()``, ``required_working_memory()``, ``resize()`` and
+``__call__`` taking the kernel's arrays in the reference's argument order
+(``utils.kernel_argument_ordering``, utils.py:203-277):
+
+* Jacobian:       ``kernel(param, phi, jac)``
+* species rates:  ``kernel(param, phi, dphi)``
+* chem utils:     ``kernel(phi, b, specific_energy, specific_heat)``
+
+Arrays are numpy float64 in the kernel's data order ('C' or 'F').  The
+calls also accept device arrays (``torch`` CUDA tensors or any object with
+``data_ptr()``), in which case no host copy is made.  There is no CPU
+fallback: a missing library raises.
+"""
+
+import ctypes
+import os
+
+import numpy as np
+
+from ..core.exceptions import HipRuntimeError, InvalidInputSpecificationException
+from ..core.enum_types import JacobianFormat, JacobianType
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                        'libpyjac_hip.so')
+
+_lib = None
+
+
+def load_library(path=None):
+    """Load ``libpyjac_hip.so`` (raises if it has not been built)."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    path = path or os.environ.get('PYJAC_HIP_LIB', LIB_PATH)
+    if not os.path.exists(path):
+        raise HipRuntimeError(-1, 'HIP library not built: {} (run '
+                                  '__graft_entry__.build() or '
+                                  'pyjac.libgen.build_library())'.format(path))
+    L = ctypes.CDLL(path)
+    P, I, S = ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t
+    L.pyjac_hip_last_error.restype = ctypes.c_char_p
+    L.pyjac_hip_create.argtypes = [P, I, ctypes.c_char, I, I, ctypes.POINTER(P)]
+    L.pyjac_hip_destroy.argtypes = [P]
+    L.pyjac_hip_resize.argtypes = [P, S, S]
+    for name in ('pyjac_hip_species_rates', 'pyjac_hip_jacobian'):
+        getattr(L, name).argtypes = [P, P, P, P]
+    L.pyjac_hip_chem_utils.argtypes = [P, P, P, P, P]
+    for name in ('pyjac_hip_species_rates_dev', 'pyjac_hip_jacobian_dev'):
+        getattr(L, name).argtypes = [P, S, P, P, P, P]
+    for name in ('pyjac_hip_num_species', 'pyjac_hip_num_reactions',
+                 'pyjac_hip_jacobian_size'):
+        getattr(L, name).argtypes = [P]
+    L.pyjac_hip_required_memory.argtypes = [P]
+    L.pyjac_hip_required_memory.restype = S
+    L.pyjac_hip_set_variant.argtypes = [P, I, I]
+    L.pyjac_hip_get_variant.argtypes = [P, I]
+    if path == LIB_PATH or _lib is None:
+        _lib = L
+    return L
+
+
+def check(status):
+    if status != 0:
+        msg = _lib.pyjac_hip_last_error().decode() if _lib else ''
+        raise HipRuntimeError(status, msg)
+
+
+def _ptr(a):
+    """Pointer of a numpy array or a device tensor; (pointer, on_device)."""
+    if isinstance(a, np.ndarray):
+        if a.dtype != np.float64:
+            raise TypeError('arrays must be float64')
+        if not (a.flags['C_CONTIGUOUS'] or a.flags['F_CONTIGUOUS']):
+            raise ValueError('arrays must be contiguous')
+        return ctypes.c_void_p(a.ctypes.data), False
+    if hasattr(a, 'data_ptr'):
+        if hasattr(a, 'dtype') and str(a.dtype) not in ('torch.float64',):
+            raise TypeError('device arrays must be float64')
+        if hasattr(a, 'is_contiguous') and not a.is_contiguous():
+            raise ValueError('device arrays must be contiguous')
+        return ctypes.c_void_p(a.data_ptr()), True
+    raise TypeError('unsupported array type {}'.format(type(a)))
+
+
+def _numel(a):
+    return a.size if isinstance(a, np.ndarray) else a.numel()
+
+
+class _KernelBase(object):
+    """Holds one handle (mechanism + options) of the C ABI."""
+
+    kernel_type = None
+
+    def __init__(self, problem_size, work_size=0, do_not_compile=False,
+                 mech=None, conp=True, order='C', jac_format=JacobianFormat.sparse,
+                 jac_type=JacobianType.exact, **kwargs):
+        if mech is None:
+            raise InvalidInputSpecificationException('mech')
+        if order not in ('C', 'F'):
+            raise InvalidInputSpecificationException('order')
+        self.lib = load_library()
+        self.mech = mech
+        self.conp = bool(conp)
+        self.order = order
+        self.jac_format = int(JacobianFormat(jac_format))
+        self.jac_type = int(JacobianType(jac_type))
+        h = ctypes.c_void_p()
+        check(self.lib.pyjac_hip_create(ctypes.cast(mech.pointer(), ctypes.c_void_p),
+                                        int(self.conp), order.encode(),
+                                        self.jac_format, self.jac_type,
+                                        ctypes.byref(h)))
+        self._h = h
+        self.problem_size = 0
+        if not kwargs.pop('testing_only_skip_compilation', False):
+            self.resize(problem_size, work_size, do_not_compile)
+
+    def __del__(self):
+        h = getattr(self, '_h', None)
+        if h is not None and h.value and self.lib is not None:
+            self.lib.pyjac_hip_destroy(h)
+            self._h = None
+
+    def species_names(self):
+        return list(self.mech.species_names)
+
+    def reaction_strings(self):
+        return list(self.mech.reaction_strings)
+
+    def num_species(self):
+        return self.lib.pyjac_hip_num_species(self._h)
+
+    def num_reactions(self):
+        return self.lib.pyjac_hip_num_reactions(self._h)
+
+    def required_working_memory(self):
+        return self.lib.pyjac_hip_required_memory(self._h)
+
+    def order_str(self):
+        return self.order
+
+    def resize(self, problem_size, work_size=0, do_not_compile=False):
+        """Resize the staging buffers for ``problem_size`` states."""
+        check(self.lib.pyjac_hip_resize(self._h, int(problem_size), int(work_size)))
+        self.problem_size = int(problem_size)
+
+    def set_variant(self, variant):
+        check(self.lib.pyjac_hip_set_variant(self._h, self._variant_slot, int(variant)))
+
+    def _run(self, host_fn, dev_fn, param, phi, out):
+        pp, dp = _ptr(param)
+        ph, dph = _ptr(phi)
+        po, do = _ptr(out)
+        if dp or dph or do:
+            if not (dp and dph and do):
+                raise ValueError('mix of host and device arrays')
+            n = _numel(param)
+            stream = _current_stream_ptr()
+            check(dev_fn(self._h, n, pp, ph, po, stream))
+            return out
+        n = _numel(param)
+        if n != self.problem_size:
+            self.resize(n)
+        check(host_fn(self._h, pp, ph, po))
+        return out
+
+
+def _current_stream_ptr():
+    try:
+        import torch
+        if torch.cuda.is_available():
+            return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    except ImportError:
+        pass
+    return ctypes.c_void_p(0)
+
+
+class PyJacobianKernel(_KernelBase):
+    """Analytical Jacobian: ``kernel(param, phi, jac)``."""
+    kernel_type = 'jacobian'
+    _variant_slot = 1
+
+    def __call__(self, param, phi, jac):
+        return self._run(self.lib.pyjac_hip_jacobian, self.lib.pyjac_hip_jacobian_dev,
+                         param, phi, jac)
+
+    def jacobian_size(self):
+        return self.lib.pyjac_hip_jacobian_size(self._h)
+
+
+class PySpeciesRatesKernel(_KernelBase):
+    """Species rates: ``kernel(param, phi, dphi)``."""
+    kernel_type = 'species_rates'
+    _variant_slot = 0
+
+    def __call__(self, param, phi, dphi):
+        return self._run(self.lib.pyjac_hip_species_rates,
+                         self.lib.pyjac_hip_species_rates_dev, param, phi, dphi)
+
+
+class PyChemUtilsKernel(_KernelBase):
+    """Thermodynamic properties: ``kernel(phi, b, specific_energy,
+    specific_heat)`` (h, cp for CONP; u, cv for CONV)."""
+    kernel_type = 'chem_utils'
+    _variant_slot = 2
+
+    def __call__(self, phi, b, specific_energy, specific_heat):
+        n = _numel(phi) // self.mech.NN
+        if n != self.problem_size:
+            self.resize(n)
+        args = [_ptr(a)[0] for a in (phi, specific_energy, specific_heat, b)]
+        check(self.lib.pyjac_hip_chem_utils(self._h, *args))
+        return b, specific_energy, specific_heat
+
+
+# names under which the reference's generated module exposes the classes
+PySpecies_RatesKernel = PySpeciesRatesKernel
+PyChem_UtilsKernel = PyChemUtilsKernel
+
+
+def bind(mech, conp=True, order='C', jac_format=JacobianFormat.sparse,
+         jac_type=JacobianType.exact):
+    """Kernel classes with the reference's constructor signature
+    ``(problem_size, work_size, do_not_compile=False)`` bound to one
+    mechanism and option set -- what the reference's generated
+    ``pyjac_<lang>`` module provides."""
+    opts = dict(mech=mech, conp=conp, order=order, jac_format=jac_format,
+                jac_type=jac_type)
+
+    def make(base):
+        class Bound(base):
+            def __init__(self, problem_size, work_size=0, do_not_compile=False, **kw):
+                kw2 = dict(opts)
+                kw2.update(kw)
+                base.__init__(self, problem_size, work_size, do_not_compile, **kw2)
+        Bound.__name__ = base.__name__
+        return Bound
+
+    return {'PyJacobianKernel': make(PyJacobianKernel),
+            'PySpeciesRatesKernel': make(PySpeciesRatesKernel),
+            'PySpecies_RatesKernel': make(PySpeciesRatesKernel),
+            'PyChemUtilsKernel': make(PyChemUtilsKernel),
+            'PyChem_UtilsKernel': make(PyChemUtilsKernel)}

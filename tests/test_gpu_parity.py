@@ -1,0 +1,82 @@
+"""HIP kernels vs the CPU checker on seeded states (through the C ABI).
+
+Tolerances follow the acceptance bar of this port: species rates within
+``|gpu - ref| <= 1e-10 |ref| + 1e-20`` element-wise, Jacobians within a
+relative Frobenius error of 1e-8 per state (pyJac's Jacobian error norm,
+functional_tester/test.py:1131-1242), both measured against the checker's
+FP64 result on the same inputs.
+"""
+
+import numpy as np
+import pytest
+
+import oracle
+from pyjac.core.states import sample_states
+from pyjac.pywrap import PyJacobianKernel, PySpeciesRatesKernel, PyChemUtilsKernel
+
+pytestmark = pytest.mark.gpu
+
+RATE_RTOL, RATE_ATOL = 1e-10, 1e-20
+JAC_FRO = 1e-8
+
+CASES = [(m, conp, order) for m in ('h2o2', 'gri30', 'test')
+         for conp in (True, False) for order in ('C', 'F')]
+
+
+def _states(mech, n, conp, order, seed=7):
+    param, phi = sample_states(mech, n, seed=seed, T_range=(900.0, 2400.0),
+                               alpha=0.5, conp=conp)
+    dev_phi = phi if order == 'C' else np.ascontiguousarray(phi.T)
+    return param, phi, dev_phi
+
+
+def _rate_err(got, ref):
+    return np.max(np.abs(got - ref) / (RATE_RTOL * np.abs(ref) + RATE_ATOL))
+
+
+@pytest.mark.parametrize('name,conp,order', CASES)
+def test_species_rates(mechs, name, conp, order):
+    mech = mechs(name)
+    n = 1000
+    param, phi, dphi_in = _states(mech, n, conp, order)
+    ref = oracle.species_rates(mech, conp, param, phi)
+    k = PySpeciesRatesKernel(n, 0, mech=mech, conp=conp, order=order)
+    out = np.zeros_like(dphi_in)
+    k(param, dphi_in, out)
+    got = out if order == 'C' else out.T
+    assert np.all(np.isfinite(got))
+    worst = _rate_err(got, ref)
+    assert worst <= 1.0, 'rates error {:.3g} x tolerance'.format(worst)
+
+
+@pytest.mark.parametrize('name,conp,order', CASES)
+@pytest.mark.parametrize('fmt', [0, 1])
+def test_jacobian(mechs, name, conp, order, fmt):
+    mech = mechs(name)
+    n = 600
+    param, phi, phi_in = _states(mech, n, conp, order)
+    dense = oracle.jacobian(mech, conp, param, phi)
+    ref = oracle.to_layout(mech, dense, order, fmt)
+    k = PyJacobianKernel(n, 0, mech=mech, conp=conp, order=order, jac_format=fmt)
+    out = np.zeros_like(ref)
+    k(param, phi_in, out)
+    assert np.all(np.isfinite(out))
+    per_state = out.reshape(n, -1) if order == 'C' else out.reshape(-1, n).T
+    ref_state = ref.reshape(n, -1) if order == 'C' else ref.reshape(-1, n).T
+    fro = np.linalg.norm(per_state - ref_state, axis=1) / np.linalg.norm(ref_state, axis=1)
+    assert fro.max() <= JAC_FRO, 'Jacobian Frobenius error {:.3g}'.format(fro.max())
+
+
+@pytest.mark.parametrize('name,conp', [('gri30', True), ('test', False)])
+def test_chem_utils(mechs, name, conp):
+    mech = mechs(name)
+    n = 512
+    param, phi = sample_states(mech, n, seed=3, conp=conp)
+    e_ref, h_ref, b_ref = oracle.chem_utils(mech, conp, phi)
+    k = PyChemUtilsKernel(n, 0, mech=mech, conp=conp, order='C')
+    b = np.zeros((n, mech.Ns))
+    e = np.zeros_like(b)
+    h = np.zeros_like(b)
+    k(phi, b, e, h)
+    for got, ref in ((e, e_ref), (h, h_ref), (b, b_ref)):
+        assert np.allclose(got, ref, rtol=1e-12, atol=1e-12 * np.abs(ref).max())

@@ -1,0 +1,153 @@
+"""Benchmark: GRI-Mech 3.0 analytical Jacobians/s and species-rate states/s
+on one MI355X.
+
+Workload (the configuration the headline metric is quoted on): GRI-Mech 3.0
+(53 species, 325 reactions), constant volume, F (state-fastest) data order,
+1,048,576 seeded states (T ~ U(1000, 2800) K, P log-uniform 1-50 atm,
+Dirichlet(0.3) mass fractions, V = 1 m^3), species rates plus the sparse
+(CCS, 1984 non-zeros) analytical Jacobian, all FP64.  A step is one pass of
+the Jacobian kernel and one pass of the species-rate kernel over the whole
+batch, inputs and outputs resident in HBM.
+
+Prints one JSON line (see the keys below).  ``--gpus`` above 1 is rejected:
+this build runs on one GPU.
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, 'pyjac-v2_amd'))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=20)
+    ap.add_argument('--warmup', type=int, default=3)
+    ap.add_argument('--states', type=int, default=1048576)
+    ap.add_argument('--cpu-seconds', type=float, default=15.0,
+                    help='approximate CPU work of the CPU-baseline sample')
+    ap.add_argument('--no-cpu', action='store_true')
+    return ap.parse_args()
+
+
+def cpu_baseline(mech, budget_s):
+    """Time the CPU checker (OpenMP, all host cores) on a bounded sample of
+    the same workload: Jacobians (which include the species rates)."""
+    import oracle
+    from pyjac.core.states import sample_states
+    threads = os.cpu_count() or 1
+    env_threads = os.environ.get('OMP_NUM_THREADS')
+    if env_threads and env_threads.isdigit():
+        threads = int(env_threads)
+    # calibrate on a small sample, then size the timed sample to the budget
+    n0 = 2048
+    param, phi = sample_states(mech, n0, seed=1234, T_range=(1000.0, 2800.0),
+                               alpha=0.3, conp=False)
+    oracle.jacobian(mech, False, param, phi, nthreads=threads)
+    t0 = time.perf_counter()
+    oracle.jacobian(mech, False, param, phi, nthreads=threads)
+    dt = time.perf_counter() - t0
+    n = int(min(262144, max(n0, n0 * budget_s / max(dt, 1e-6))))
+    param, phi = sample_states(mech, n, seed=1235, T_range=(1000.0, 2800.0),
+                               alpha=0.3, conp=False)
+    t0 = time.perf_counter()
+    oracle.jacobian(mech, False, param, phi, nthreads=threads)
+    dt = time.perf_counter() - t0
+    return {'value': n / dt, 'unit': 'Jacobians/s', 'cores': threads, 'kind': 'port',
+            'sample': '{} GRI-3.0 CONV states, analytical Jacobian incl. species rates, '
+                      'C++ restatement with OpenMP over states'.format(n)}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print('bench.py runs on one GPU only (this build has no multi-GPU path); '
+              'got --gpus {}'.format(args.gpus), file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from pyjac._paths import mechanism_files
+    from pyjac.core.mech_tables import MechTables
+    from pyjac.core.states import sample_states
+    from pyjac.pywrap import PyJacobianKernel, PySpeciesRatesKernel
+
+    dev = torch.device('cuda:0')
+    mech = MechTables.from_file(*mechanism_files('gri30'))
+    n = args.states
+    param, phi = sample_states(mech, n, seed=20240, T_range=(1000.0, 2800.0),
+                               alpha=0.3, conp=False, order='F')
+    d_param = torch.from_numpy(param).to(dev)
+    d_phi = torch.from_numpy(phi).to(dev)
+    d_dphi = torch.empty_like(d_phi)
+    d_jac = torch.empty((mech.nnz, n), dtype=torch.float64, device=dev)
+    jk = PyJacobianKernel(n, 0, mech=mech, conp=False, order='F', jac_format=1)
+    rk = PySpeciesRatesKernel(n, 0, mech=mech, conp=False, order='F')
+
+    for _ in range(args.warmup):
+        jk(d_param, d_phi, d_jac)
+        rk(d_param, d_phi, d_dphi)
+    torch.cuda.synchronize()
+
+    stream = torch.cuda.current_stream()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True),
+           torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for e0, e1, e2 in ev:
+        e0.record(stream)
+        jk(d_param, d_phi, d_jac)
+        e1.record(stream)
+        rk(d_param, d_phi, d_dphi)
+        e2.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    jac_ms = float(np.mean([a.elapsed_time(b) for a, b, _ in ev]))
+    rates_ms = float(np.mean([b.elapsed_time(c) for _, b, c in ev]))
+
+    jac_states_s = n / (jac_ms * 1e-3)
+    rates_states_s = n / (rates_ms * 1e-3)
+    bytes_per_state = (mech.NN + 1) * 8 + mech.nnz * 8
+    achieved = bytes_per_state * n / (jac_ms * 1e-3) / 1e9
+    ms_per_step = wall * 1e3 / args.steps
+
+    line = {
+        'metric': 'GRI-30 Jacobians/s & species-rate states/s, 1 GPU; surrogate speedup @1e-3 err',
+        'value': jac_states_s,
+        'unit': 'Jacobians/s',
+        'n_gpus': 1,
+        'steps': args.steps,
+        'warmup': args.warmup,
+        'ms_per_step': ms_per_step,
+        'higher_is_better': True,
+        'scaling': 'weak',
+        'vs_baseline': None,
+        'dtype': 'f64',
+        'data': 'synthetic seeded states',
+        'config': {'workload': 'gri30_conv_f_sparse_jacobian_and_rates',
+                   'mechanism': 'GRI-Mech 3.0 (53 species, 325 reactions)',
+                   'states': n, 'conp': False, 'order': 'F',
+                   'jac_format': 'sparse CCS', 'jac_nnz': mech.nnz},
+        'species_rates_states_per_s': rates_states_s,
+        'jacobian_ms': jac_ms,
+        'species_rates_ms': rates_ms,
+        'roofline': {'bound': 'hbm', 'achieved': achieved, 'peak': HBM_PEAK_GBS,
+                     'unit': 'GB/s', 'frac': achieved / HBM_PEAK_GBS, 'traffic': None,
+                     'kernel': 'jacobian_kernel',
+                     'algorithmic_bytes_per_state': bytes_per_state},
+    }
+    if not args.no_cpu:
+        line['cpu_baseline'] = cpu_baseline(mech, args.cpu_seconds)
+    print(json.dumps(line))
+
+
+if __name__ == '__main__':
+    main()

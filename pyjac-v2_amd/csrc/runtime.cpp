@@ -64,6 +64,8 @@ struct pyjac_handle {
     int device = 0;
     int n_cu = 256;
     DevMech m{};
+    pjk_host::WaveTables wt{};
+    bool wave_ok_rates = false, wave_ok_jac = false;
     char* d_blob = nullptr;
     // workspace of the persistent kernels
     double* d_ws = nullptr;
@@ -112,9 +114,26 @@ int ensure_ws(pyjac_handle* h, size_t bytes) {
     return PYJAC_OK;
 }
 
+// persistent grid of one-wave workgroups for the wave-per-state kernels
+int wave_grid(const pyjac_handle* h, size_t lds_bytes, long n) {
+    long per_cu = (long)((160 * 1024) / std::max<size_t>(lds_bytes, 1));
+    per_cu = std::max(1L, std::min(per_cu, 32L));
+    const char* env = getenv("PYJAC_HIP_WAVES_PER_CU");
+    if (env && atoi(env) > 0) per_cu = atoi(env);
+    long G = (long)h->n_cu * per_cu;
+    G = std::min(G, std::max(8L, (n + 7) / 8 * 8));
+    return (int)std::max(8L, G / 8 * 8);
+}
+
 int run_rates(pyjac_handle* h, long n, const double* param, const double* phi, double* dphi,
               hipStream_t s) {
     if (n <= 0) return PYJAC_OK;
+    if (h->wave_ok_rates && h->variant[0] != 1) {
+        size_t lds = pjk_host::wave_lds_bytes(h->m, false, 0);
+        HIP_TRY(pjk_host::launch_rates_wave(h->m, h->conp, h->order, n, param, phi, dphi, h->wt,
+                                            wave_grid(h, lds, n), s));
+        return PYJAC_OK;
+    }
     size_t per = pjk_host::ws_doubles_per_state(h->m, false);
     int grid = pick_grid(h, per, h->variant[0], n);
     int rc = ensure_ws(h, per * (size_t)grid * h->block * sizeof(double));
@@ -127,6 +146,12 @@ int run_rates(pyjac_handle* h, long n, const double* param, const double* phi, d
 int run_jac(pyjac_handle* h, long n, const double* param, const double* phi, double* jac,
             hipStream_t s) {
     if (n <= 0) return PYJAC_OK;
+    if (h->wave_ok_jac && h->variant[1] != 1) {
+        size_t lds = pjk_host::wave_lds_bytes(h->m, true, h->m.NN * h->m.NN);
+        HIP_TRY(pjk_host::launch_jacobian_wave(h->m, h->conp, h->order, n, param, phi, jac, h->wt,
+                                               wave_grid(h, lds, n), s));
+        return PYJAC_OK;
+    }
     size_t per = pjk_host::ws_doubles_per_state(h->m, true);
     int grid = pick_grid(h, per, h->variant[1], n);
     int rc = ensure_ws(h, per * (size_t)grid * h->block * sizeof(double));
@@ -193,9 +218,9 @@ int pyjac_hip_create(const pyjac_mech_tables* mt, int conp, char order, int jac_
     if (order != 'C' && order != 'F') return fail(PYJAC_ERR_ARG, "order must be 'C' or 'F'");
     if (jac_format != PYJAC_JAC_FULL && jac_format != PYJAC_JAC_SPARSE)
         return fail(PYJAC_ERR_ARG, "bad jac_format %d", jac_format);
-    if (jac_type != PYJAC_JAC_EXACT && jac_type != PYJAC_JAC_APPROXIMATE)
-        return fail(PYJAC_ERR_UNSUPPORTED, "jac_type %d not supported by the HIP backend",
-                    jac_type);
+    if (jac_type != PYJAC_JAC_EXACT)
+        return fail(PYJAC_ERR_UNSUPPORTED,
+                    "jac_type %d not supported by the HIP backend (exact only)", jac_type);
     const int Ns = mt->n_species, Nr = mt->n_reactions, NN = Ns + 1;
     if (Ns < 2 || Nr < 1) return fail(PYJAC_ERR_ARG, "empty mechanism");
     if (mt->n_cheb && (mt->cheb_max_p > 16 || mt->cheb_max_t > 16))
@@ -230,6 +255,43 @@ int pyjac_hip_create(const pyjac_mech_tables* mt, int conp, char order, int jac_
         for (int row = 0; row < NN; ++row)
             for (int q = mt->crs_row_ptr[row]; q < mt->crs_row_ptr[row + 1]; ++q)
                 jac_pos.push_back(row * NN + mt->crs_col_ind[q]);
+    }
+
+    // packed reaction records of the wave-per-state kernels
+    struct I4 { int x, y, z, w; };
+    struct D4 { double x, y, z, w; };
+    std::vector<I4> rx_i(Nr);
+    std::vector<D4> rx_d(Nr);
+    std::vector<int> rx_s((size_t)8 * Nr, 0);
+    bool packable = Ns < 32767;
+    for (int r = 0; r < Nr; ++r) {
+        int kind, si = mt->simple_mask[r];
+        D4 d{0, 0, 0, 0};
+        if (si >= 0) {
+            kind = mt->simple_form[si];
+            d = D4{mt->simple_A[si], mt->simple_b[si], mt->simple_Ta[si], 0.0};
+        } else if (mt->plog_mask[r] >= 0) {
+            kind = 5;
+            d.w = mt->plog_mask[r];
+        } else {
+            kind = 6;
+            d.w = mt->n_plog + mt->cheb_mask[r];
+        }
+        int nsum = mt->nu_sum[r];
+        int cnt = mt->rxn_off[r + 1] - mt->rxn_off[r];
+        if (nsum < -100 || nsum > 100 || cnt > pjk_host::WAVE_MAXSP || cnt < 1) packable = false;
+        rx_i[r] = I4{kind | ((nsum + 128) << 8), mt->rev_mask[r], mt->thd_mask[r], cnt};
+        rx_d[r] = d;
+        for (int q = 0; q < 8; ++q) {
+            int idx = mt->rxn_off[r] + q;
+            if (q < cnt && q < pjk_host::WAVE_MAXSP) {
+                int nr = mt->nu_reac[idx], np = mt->nu_prod[idx];
+                if (nr < 0 || nr > 255 || np < 0 || np > 127) packable = false;
+                rx_s[(size_t)8 * r + q] = mt->rxn_spec[idx] | (nr << 16) | (np << 24);
+            } else {
+                rx_s[(size_t)8 * r + q] = Ns;  // padding slot, nu = 0
+            }
+        }
     }
 
     Packer P;
@@ -303,6 +365,9 @@ int pyjac_hip_create(const pyjac_mech_tables* mt, int conp, char order, int jac_
     PACK(fall_order, fall_order.data(), fall_order.size());
     PACK(jac_pos, jac_pos.data(), jac_pos.size());
 #undef PACK
+    size_t off_rx_i = P.add(rx_i.data(), rx_i.size());
+    size_t off_rx_d = P.add(rx_d.data(), rx_d.size());
+    size_t off_rx_s = P.add(rx_s.data(), rx_s.size());
     hipError_t e = hipMalloc(&h->d_blob, P.bytes.size());
     if (e == hipSuccess)
         e = hipMemcpy(h->d_blob, P.bytes.data(), P.bytes.size(), hipMemcpyHostToDevice);
@@ -312,6 +377,13 @@ int pyjac_hip_create(const pyjac_mech_tables* mt, int conp, char order, int jac_
         return fail(PYJAC_ERR_HIP, "table upload failed: %s", hipGetErrorString(e));
     }
     for (auto& s : slots) *s.dst = h->d_blob + s.off;
+    h->wt.rx_i = h->d_blob + off_rx_i;
+    h->wt.rx_d = h->d_blob + off_rx_d;
+    h->wt.rx_s = h->d_blob + off_rx_s;
+    // the wave-per-state kernels keep a state's arrays in 64 KiB of LDS at most
+    const size_t lds_cap = 64 * 1024;
+    h->wave_ok_rates = packable && pjk_host::wave_lds_bytes(m, false, 0) <= lds_cap;
+    h->wave_ok_jac = packable && pjk_host::wave_lds_bytes(m, true, NN * NN) <= lds_cap;
     *out = h;
     return PYJAC_OK;
 }

@@ -42,4 +42,47 @@ hipError_t launch_jacobian_wave(const DevMech& m, int conp, char order, long n,
                                 const double* param, const double* phi, double* jac,
                                 const WaveTables& wt, int grid, hipStream_t stream);
 
+// ---- block-per-state kernels (pyjac_block.hip): WPS waves per state,
+// per-state arrays and the dense Jacobian accumulator in LDS
+struct BlockTables {
+    const void* rec_i;       // int4 [Nr] reaction records sorted by structure
+    const void* rec_d;       // double4 [Nr]
+    const void* rec_s;       // int [8 Nr] species slots (slot 6: a row species)
+    const int* grp_flags;    // [ceil(Nr / 64)] flags of each group of 64 records
+    const int* thd_order;    // [n_thd] third-body entries: plain, Lindemann, SRI, Troe
+    const double* thd_mod;   // [n_thd] efficiency of the bath gas in the entry's d/dT, d/dE
+    const double* effns;     // [n_thd] d(ci)/dn_j through the bath gas
+    const int* thd_pairs;    // [n_thd][8] per species slot: coefficient pair or -1
+    const void* combo;       // int4 [n_combo]: pair, dense index, efficiency (2 ints)
+    const void* out_pos;     // int2 [jac_nnz]: dense index, row of the bath-gas sum or -1
+    int n_pairs, n_combo;
+};
+
+// LDS offsets (in doubles) of one workgroup's arrays
+struct BlockLds {
+    int conc, nmol, b, db, wdot, energy, heat, dheat, U;
+    int ci, nfac, dTf, dEf, Cp, mw, ksp, dkT, dkP, jac, total;
+};
+
+// everything the block kernels read besides the states: one copy in device
+// memory per handle and kernel, passed by pointer so the tables are fetched
+// through the scalar cache when used
+struct BlockArgs {
+    DevMech m;
+    BlockTables bt;
+    BlockLds L;
+};
+
+BlockLds block_lds(const DevMech& m, bool jac, int n_pairs);
+size_t block_lds_bytes(const DevMech& m, bool jac, int n_pairs);
+
+// one workgroup of 64 * wps threads per state
+hipError_t launch_rates_block(const DevMech& m, const BlockArgs* args, const BlockLds& L, int wps,
+                              int conp, char order, long n, const double* param,
+                              const double* phi, double* dphi, hipStream_t stream);
+
+hipError_t launch_jacobian_block(const DevMech& m, const BlockArgs* args, const BlockLds& L,
+                                 int wps, int conp, char order, long n, const double* param,
+                                 const double* phi, double* jac, hipStream_t stream);
+
 }  // namespace pjk_host

@@ -1,0 +1,1196 @@
+// pyjac_block.hip -- block-per-state species-rate and Jacobian kernels (gfx950).
+//
+// One workgroup of WPS wavefronts (WPS = 1, 2, 4 or 8) evaluates one
+// thermochemical state at a time and walks the batch persistently.  All
+// arrays a state needs at random access -- concentrations, Gibbs terms,
+// production rates, third-body / falloff factors and, for the Jacobian, a
+// dense (Ns+1)^2 accumulator -- live in the workgroup's LDS, so HBM sees only
+// the state in and the result out.
+//
+// A state is processed in phases separated by workgroup barriers:
+//   1  concentrations and NASA-7 thermodynamics (threads over species);
+//      the bath-gas concentration by difference, summed in the reference's
+//      sequential order (the difference cancels when the bath gas is rare)
+//   2  third-body concentrations, falloff blending and every per-third-body
+//      derivative factor; PLOG / Chebyshev rate constants and their
+//      temperature / pressure derivatives (threads over those reactions,
+//      third-body entries ordered by kind so a wave runs one code path)
+//   3  reactions (threads over reactions, records sorted on the host by
+//      structure so a wave's lanes follow the same branches): rate
+//      constants, rates of progress, production rates and the per-reaction
+//      Jacobian contributions, scattered into LDS with ds_add_f64
+//   4  third-body columns (waves over third-body reactions, lanes over
+//      species columns), temperature rate
+//   5  temperature / extra-variable rows (lanes over columns, waves over
+//      row blocks) and the 2x2 corner
+//   6  write-out of the stored Jacobian entries, in C or F order
+//
+// Rate constants use the specialised Arrhenius forms of the reference
+// (rate_subs.py:2803-2870) evaluated branch-free; every formula cites the
+// reference kernel it follows.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "device_mech.h"
+#include "kernels.h"
+
+namespace pjb {
+
+constexpr double RU = 8314.4621;
+constexpr double PA = 101325.0;
+constexpr double EXP_MAX = 690.775527898;
+constexpr double SMALL = 1e-300;
+constexpr double T_MIN = 100.0, T_MAX = 10000.0;
+constexpr double V_MIN = 1e-30;
+constexpr double LN10 = 2.302585092994046;
+constexpr double LN10SQ = LN10 * LN10;
+constexpr double RU_INV = 1.0 / RU;
+constexpr int MAXSP = pjk_host::WAVE_MAXSP;
+
+using pjk_host::BlockTables;
+using pjk_host::BlockLds;
+using pjk_host::BlockArgs;
+
+// wave-uniform load through the scalar data path
+template <class T>
+__device__ __forceinline__ T sld(const T* p, long i) {
+    return ((const __attribute__((address_space(4))) T*)p)[i];
+}
+
+// fast_powi of the reference (loopy_utils/preambles_and_manglers.py:123-174):
+// a left-to-right product chain, here without data-dependent branches for
+// exponents up to 3
+__device__ __forceinline__ double powi(double val, int p) {
+    double v = p < 0 ? 1.0 / val : val;
+    int e = p < 0 ? -p : p;
+    double r = e >= 1 ? v : 1.0;
+    r = e >= 2 ? r * v : r;
+    r = e >= 3 ? r * v : r;
+    for (int i = 3; i < e; ++i) r *= v;
+    return r;
+}
+
+__device__ __forceinline__ double clampT(double T) { return fmin(T_MAX, fmax(T_MIN, T)); }
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int s = 32; s >= 1; s >>= 1) v += __shfl_xor(v, s);
+    return v;
+}
+
+// Arrhenius forms, rate_subs.py:2803-2870 (A holds log A for forms 2-4)
+__device__ __forceinline__ double arrhenius(int form, double A, double b, double Ta, double Tc,
+                                            double logT, double Tinv) {
+    switch (form) {
+        case 0: return A;
+        case 1: return A * powi(Tc, (int)b);
+        case 3: return exp(fmin(EXP_MAX, A + logT * b));
+        case 4: return exp(fmin(EXP_MAX, A - Ta * Tinv));
+        default: return exp(fmin(EXP_MAX, A + logT * b + -1.0 * Ta * Tinv));
+    }
+}
+
+// Per-reaction record (runtime.cpp builds them, sorted by structure):
+//   rec_i.x = kind | (nu_sum + 128) << 8   kind: 0-4 Arrhenius form,
+//                                           5 PLOG, 6 Chebyshev
+//   rec_i.y = 1 if reversible, rec_i.z = third-body index or -1,
+//   rec_i.w = number of species slots (<= MAXSP)
+//   rec_d   = A (log A for forms 2-4), b, Ta, PLOG/Chebyshev rate index
+//   rec_s[8 r + q] = species | nu_reac << 16 | nu_prod << 24, padded with
+//                    species Ns (concentration 1, Gibbs term 0); slot 6
+//                    holds a species of the reaction with a Jacobian row
+//                    (the target of the lane's zero-valued atomics)
+// Records are processed 64 at a time; grp_flags[r / 64] describes the group
+// (largest species count, and which optional parts any of its reactions
+// has) so that a wave skips whole blocks with uniform branches.
+struct Rec {
+    int kind, nsum, rev, thd, cnt, kidx, home;
+    double A, b, Ta;
+    int sp[MAXSP];  // packed species slots
+    __device__ int spec(int q) const { return sp[q] & 0xffff; }
+    __device__ int nur(int q) const { return (sp[q] >> 16) & 0xff; }
+    __device__ int nup(int q) const { return (sp[q] >> 24) & 0xff; }
+    __device__ int nu(int q) const { return nup(q) - nur(q); }
+};
+
+__device__ __forceinline__ Rec load_rec(const BlockTables& bt, int r) {
+    Rec x;
+    const int4 ri = ((const int4*)bt.rec_i)[r];
+    const double4 d = ((const double4*)bt.rec_d)[r];
+    const int4 s0 = ((const int4*)bt.rec_s)[2 * r], s1 = ((const int4*)bt.rec_s)[2 * r + 1];
+    x.kind = ri.x & 0xff;
+    x.nsum = (ri.x >> 8) - 128;
+    x.rev = ri.y;
+    x.thd = ri.z;
+    x.cnt = ri.w;
+    x.A = d.x;
+    x.b = d.y;
+    x.Ta = d.z;
+    x.kidx = (int)d.w;
+    x.sp[0] = s0.x; x.sp[1] = s0.y; x.sp[2] = s0.z; x.sp[3] = s0.w;
+    x.sp[4] = s1.x; x.sp[5] = s1.y;
+    x.home = s1.z;
+    return x;
+}
+
+// per-state values shared by all threads of the workgroup
+struct State {
+    double T, P, V, Tc, logT, Tinv, logP;
+    double Traw_inv, rt_inv, V_inv;
+};
+
+typedef __attribute__((address_space(3))) double lds_double;
+
+__device__ __forceinline__ void lds_add(lds_double* p, double v) {
+    __builtin_amdgcn_ds_atomic_fadd_f64(p, v);
+}
+
+// Everything a phase needs besides the state: table pointers (read-only
+// device memory), the LDS layout and the sizes, the latter two held by
+// value so that no LDS store forces them to be fetched again
+struct Ctx {
+    const DevMech& m;
+    const BlockTables& bt;
+    const BlockLds L;
+    lds_double* sm;
+    int conp;
+    int Ns, NN, ns;
+    __device__ lds_double& conc(int k) { return sm[L.conc + k]; }
+    __device__ lds_double& nmol(int k) { return sm[L.nmol + k]; }
+    __device__ lds_double& b(int k) { return sm[L.b + k]; }
+    __device__ lds_double& db(int k) { return sm[L.db + k]; }
+    __device__ lds_double& wdot(int k) { return sm[L.wdot + k]; }
+    __device__ lds_double& energy(int k) { return sm[L.energy + k]; }
+    __device__ lds_double& heat(int k) { return sm[L.heat + k]; }
+    __device__ lds_double& dheat(int k) { return sm[L.dheat + k]; }
+    __device__ lds_double& U(int k) { return sm[L.U + k]; }
+    __device__ lds_double& ci(int t) { return sm[L.ci + t]; }
+    __device__ lds_double& nfac(int t) { return sm[L.nfac + t]; }
+    __device__ lds_double& dTf(int t) { return sm[L.dTf + t]; }
+    __device__ lds_double& dEf(int t) { return sm[L.dEf + t]; }
+    __device__ lds_double& Cp(int p) { return sm[L.Cp + p]; }
+    __device__ lds_double& mw(int k) { return sm[L.mw + k]; }
+    __device__ lds_double& ksp(int i) { return sm[L.ksp + i]; }
+    __device__ lds_double& dkT(int i) { return sm[L.dkT + i]; }
+    __device__ lds_double& dkP(int i) { return sm[L.dkP + i]; }
+    __device__ lds_double* J() { return sm + L.jac; }
+};
+
+__device__ __forceinline__ Ctx make_ctx(const BlockArgs* A, double* smem, int conp) {
+    return Ctx{A->m, A->bt, A->L, (lds_double*)smem, conp, A->m.Ns, A->m.NN, A->m.ns};
+}
+
+// ------------------------------------------------------------- phase 1
+// get_concentrations (rate_subs.py:597-706) and the NASA-7 properties
+// (rate_subs.py:3281-3388); `phi(i)` reads entry i of this state
+template <bool JAC, int NT, class PhiF>
+__device__ __forceinline__ void phase_species(Ctx& c, const State& st, int tid, PhiF phi) {
+    const DevMech& m = c.m;
+    const int Ns = m.Ns;
+    for (int k = tid; k < Ns; k += NT) {
+        if (k < Ns - 1) {
+            double n = fmax(SMALL, phi(2 + k));
+            c.nmol(k) = n;
+            c.conc(k) = n * st.V_inv;
+        }
+        const double T = st.Tc;
+        const double* a = (T < m.T_mid[k]) ? m.a_lo + 7 * k : m.a_hi + 7 * k;
+        const double a0 = a[0], a1 = a[1], a2 = a[2], a3 = a[3], a4 = a[4], a5 = a[5], a6 = a[6];
+        c.b(k) = T * (T * (T * (T * a4 / 20.0 + a3 / 12.0) + a2 / 6.0) + a1 / 2.0) +
+                 (a0 + -1.0) * st.logT + -1.0 * a0 + a6 + -1.0 * a5 * st.Tinv;
+        const double x = T * (T * (T * (T * (T * a4 / 5.0 + a3 / 4.0) + a2 / 3.0) + a1 / 2.0) + a0);
+        const double y = T * (T * (T * (T * a4 + a3) + a2) + a1) + a0;
+        c.energy(k) = RU * (c.conp ? x + a5 : x - T + a5);
+        c.heat(k) = RU * (c.conp ? y : y - 1.0);
+        c.wdot(k) = 0.0;
+        if (JAC) {
+            c.U(k) = 0.0;
+            c.dheat(k) = RU * (T * (T * (4.0 * T * a4 + 3.0 * a3) + 2.0 * a2) + a1);
+            c.db(k) = T * (T * (T * a4 / 5.0 + a3 / 4.0) + a2 / 3.0) + a1 / 2.0 +
+                      st.Tinv * (a0 + -1.0 + a5 * st.Tinv);
+        }
+    }
+}
+
+// bath-gas concentration by difference, summed in the reference's
+// sequential order (rate_subs.py:665-706); every wave computes it and its
+// lane 0 stores it (all waves store the same value)
+__device__ __forceinline__ void bath_gas(Ctx& c, const State& st, int lane) {
+    const int Ns = c.m.Ns;
+    double nsum = 0.0;
+    for (int k = 0; k < Ns - 1; ++k) nsum = nsum + c.nmol(k);
+    if (lane == 0)
+        c.conc(Ns - 1) = __dadd_rn(st.P / (RU * st.Tc), __dmul_rn(-1.0 * fmax(SMALL, nsum), st.V_inv));
+}
+
+// ------------------------------------------------------------- phase 2
+// third-body concentration, falloff blending and the derivative factors of
+// third-body entry t:
+//   ci   pressure modification (get_thd_body_concs, rate_subs.py:1788-1901;
+//        falloff: rate_subs.py:2251-2639, :1510-1624)
+//   nfac multiplier of the third-body columns d(ci)/dn_j
+//        (create_jacobian.py:3780-3890; 1 for plain third bodies)
+//   dTf, dEf  d(ci)/dT, d(ci)/dE per unit rate of progress
+//        (create_jacobian.py:362-882 and :2117-2588)
+template <bool JAC>
+__device__ void thd_task(Ctx& c, const State& st, int t) {
+    const DevMech& m = c.m;
+    const int conp = c.conp;
+    const int tt = m.thd_type[t];
+    const int not_spec = tt != 2;
+    double thd = st.P * (double)not_spec / (RU * st.T);
+    const int e1 = m.thd_off[t + 1];
+    for (int s = m.thd_off[t]; s < e1; ++s)
+        thd = thd + (m.thd_eff[s] + -1.0 * not_spec) * c.conc(m.thd_spec[s]);
+    const int r = m.thd_map[t];
+    const int f = m.fall_mask[r];
+    const double T = st.T, Tinv = st.Traw_inv;
+    const double modT = c.bt.thd_mod[t];
+    const double modE = (conp && tt == 3) ? 0.0 : modT;
+    if (f < 0) {
+        c.ci(t) = thd;
+        if (JAC) {
+            c.nfac(t) = 1.0;
+            c.dTf(t) = -1.0 * st.P * modT * RU_INV * Tinv * Tinv * st.V;
+            if (conp) {
+                double mo = modE;
+                if (tt != 3) mo = mo * st.P * st.rt_inv - thd;
+                c.dEf(t) = mo;
+            } else {
+                c.dEf(t) = modE * st.rt_inv * st.V;
+            }
+        }
+        return;
+    }
+    // falloff / chemically activated
+    const int si = m.simple_mask[r];
+    const double kf = arrhenius(m.simple_form[si], m.simple_A[si], m.simple_b[si], m.simple_Ta[si],
+                                st.Tc, st.logT, st.Tinv);
+    const double kff = arrhenius(m.fall_form[f], m.fall_A[f], m.fall_b[f], m.fall_Ta[f], st.Tc,
+                                 st.logT, st.Tinv);
+    const int ftype = m.fall_type[f];
+    const double kinf = ftype ? kff : kf;
+    const double k0 = ftype ? kf : kff;
+    const double Pr = thd * k0 / kinf;
+    const int blend = m.fall_blend[f];
+    double Fi = 1.0, Fc = 0.0, A = 0.0, B = 0.0, X = 0.0;
+    if (blend == 2) {
+        const int i = m.fall_param[f];
+        const double a = m.troe_a[i];
+        const double logPr = log10(fmax(SMALL, Pr));
+        Fc = a * exp(fmin(EXP_MAX, -1.0 * st.Tc * m.troe_T1inv[i])) +
+             (1.0 + -1.0 * a) * exp(fmin(EXP_MAX, -1.0 * st.Tc * m.troe_T3inv[i]));
+        const double T2 = m.troe_T2[i];
+        if (T2 != 0.0) Fc = Fc + exp(fmin(EXP_MAX, -1.0 * T2 / st.Tc));
+        const double logFc = log10(fmax(SMALL, Fc));
+        B = -1.1762 * logFc + -1.0 * 0.14 * logPr + 0.806;
+        A = -0.67 * logFc + logPr + -0.4;
+        Fi = pow(fmax(SMALL, Fc), 1.0 / (A * A / (B * B) + 1.0));
+    } else if (blend == 3) {
+        const int i = m.fall_param[f];
+        const double logPr = log10(fmax(SMALL, Pr));
+        X = 1.0 / (logPr * logPr + 1.0);
+        const double base = m.sri_a[i] * exp(fmin(EXP_MAX, -m.sri_b[i] * st.Tinv)) +
+                            exp(fmin(EXP_MAX, -st.Tc / m.sri_c[i]));
+        Fi = pow(base, X);
+        const double d = m.sri_d[i], e = m.sri_e[i];
+        if (d != 1.0) Fi = Fi * d;
+        if (e != 0.0) Fi = Fi * pow(st.Tc, e);
+    }
+    double ci = Fi / (1.0 + Pr);
+    if (!ftype) ci = ci * Pr;
+    c.ci(t) = ci;
+    if (!JAC) return;
+
+    // d(ci)/dn_j factor, create_jacobian.py:3780-3890
+    {
+        double dFi = 0.0;
+        if (blend == 2) {
+            dFi = A * A + B * B;
+            dFi = -2.0 * A * B * (0.14 * A + B) * log(fmax(SMALL, Fc)) /
+                  (fmax(SMALL, Pr) * dFi * dFi * LN10);
+        } else if (blend == 3) {
+            const int i = m.fall_param[f];
+            const double lv = log(fmax(SMALL, m.sri_a[i] * exp(fmin(EXP_MAX, -m.sri_b[i] / T)) +
+                                                  exp(fmin(EXP_MAX, -T / m.sri_c[i]))));
+            dFi = -2.0 * X * X * lv * log(fmax(SMALL, Pr)) / (fmax(SMALL, Pr) * LN10SQ);
+        }
+        double Fi_fac = dFi;
+        if (!ftype) Fi_fac = Pr * Fi_fac + 1.0;
+        c.nfac(t) = k0 * (Fi * Fi_fac + -1.0 * ci) / (kinf * (Pr + 1.0));
+    }
+    // d(ci)/dT, create_jacobian.py:362-882
+    {
+        const double thd_fac = -1.0 * st.P * modT * RU_INV * Tinv * Tinv;
+        double beta_0, beta_inf, Ta_0, Ta_inf;
+        if (ftype) {
+            beta_0 = m.simple_b[si]; Ta_0 = m.simple_Ta[si];
+            beta_inf = m.fall_b[f]; Ta_inf = m.fall_Ta[f];
+        } else {
+            beta_0 = m.fall_b[f]; Ta_0 = m.fall_Ta[f];
+            beta_inf = m.simple_b[si]; Ta_inf = m.simple_Ta[si];
+        }
+        const double theta_Pr = Tinv * (beta_0 - beta_inf + (Ta_0 - Ta_inf) * Tinv);
+        const double theta_no_Pr = thd_fac * k0 / kinf;
+        double dFi = 0.0;
+        if (blend == 2) {
+            const int i = m.fall_param[f];
+            const double a = m.troe_a[i], T1 = m.troe_T1inv[i], T2 = m.troe_T2[i],
+                         T3 = m.troe_T3inv[i];
+            const double dFcent = -1.0 * a * T1 * exp(fmin(EXP_MAX, -1.0 * T * T1)) +
+                                  (a + -1.0) * T3 * exp(fmin(EXP_MAX, -1.0 * T * T3)) +
+                                  T2 * Tinv * Tinv * exp(fmin(EXP_MAX, -1.0 * T2 * Tinv));
+            const double logFcent = log(fmax(SMALL, Fc));
+            const double absq = A * A + B * B;
+            const double absqsq = absq * absq;
+            dFi = -1.0 * B *
+                  (2.0 * A * Fc * (0.14 * A + B) * (Pr * theta_Pr + theta_no_Pr) * logFcent +
+                   Pr * dFcent * (2.0 * A * (1.1762 * A + -1.0 * 0.67 * B) * logFcent +
+                                  -1.0 * B * absq * LN10)) /
+                  (Fc * fmax(SMALL, Pr) * absqsq * LN10);
+        } else if (blend == 3) {
+            const int i = m.fall_param[f];
+            const double a = m.sri_a[i], b = m.sri_b[i], e_ = m.sri_e[i];
+            const double cinv = 1.0 / m.sri_c[i];
+            const double ec = exp(fmin(EXP_MAX, -T * cinv));
+            const double eb = exp(fmin(EXP_MAX, -b * Tinv));
+            const double lv = log(fmax(SMALL, a * eb + ec));
+            dFi = -X * (ec * cinv - a * b * Tinv * Tinv * eb) / (a * eb + ec) + e_ * Tinv -
+                  2 * X * X * lv * (Pr * theta_Pr + theta_no_Pr) * log(fmax(SMALL, Pr)) /
+                      (fmax(SMALL, Pr) * LN10SQ);
+        }
+        double d = ci * (-1.0 * (Pr * theta_Pr + theta_no_Pr) / (Pr + 1.0) + dFi);
+        if (!ftype) d = d + theta_Pr * ci + Fi * theta_no_Pr / (Pr + 1.0);
+        c.dTf(t) = d * st.V;
+    }
+    // d(ci)/dE, create_jacobian.py:2117-2588
+    {
+        const int not_unity = tt != 3;
+        double mod = conp ? modE * st.P * st.rt_inv * k0 / kinf : modE * st.rt_inv * k0 / kinf;
+        const double theta = conp ? (mod + -1.0 * Pr * not_unity) : mod;
+        double dFi = 0.0;
+        if (blend == 2) {
+            double absqsq = A * A + B * B;
+            absqsq = absqsq * absqsq;
+            dFi = -2.0 * A * B * log(fmax(SMALL, Fc)) * (0.14 * A + B) * theta /
+                  (fmax(SMALL, Pr) * absqsq * LN10);
+        } else if (blend == 3) {
+            const int i = m.fall_param[f];
+            const double a = m.sri_a[i], b = m.sri_b[i], cinv = 1.0 / m.sri_c[i];
+            double inner;
+            if (conp) {
+                const double exp_cinv = exp(fmin(EXP_MAX, T * cinv));
+                const double exp_b = exp(fmin(EXP_MAX, b * Tinv));
+                const double exp_bc = exp(fmin(EXP_MAX, -b * Tinv - T * cinv));
+                inner = log(fmax(SMALL, (a * exp_cinv + exp_b) * exp_bc));
+            } else {
+                const double exp_b = exp(fmin(EXP_MAX, -b * Tinv));
+                const double exp_cinv = exp(fmin(EXP_MAX, -T * cinv));
+                inner = log(fmax(SMALL, a * exp_b + exp_cinv));
+            }
+            dFi = -2 * X * X * theta * inner * log(fmax(SMALL, Pr)) / (fmax(SMALL, Pr) * LN10SQ);
+        }
+        const double num = conp ? (-mod + Pr * not_unity) : -mod;
+        double d = ci * (num / (Pr + 1) + dFi);
+        if (!ftype) {
+            if (conp) d = d + Fi * mod / (Pr + 1) + -1.0 * not_unity * ci;
+            else d = d + Fi * mod / (Pr + 1);
+        }
+        c.dEf(t) = conp ? d : d * st.V;
+    }
+}
+
+__device__ __forceinline__ double plog_par(const DevMech& m, int q, int p, int k) {
+    return m.plog_params[((size_t)q * m.n_plog + p) * m.plog_max_p + k];
+}
+
+// PLOG rate constant (rate_subs.py:2084-2248) and, for the Jacobian,
+// d ln k / dT (create_jacobian.py:2823-2976) and d ln k / dP
+// (create_jacobian.py:1158-1300)
+template <bool JAC>
+__device__ void plog_task(Ctx& c, const State& st, int p) {
+    const DevMech& m = c.m;
+    const int mp = m.plog_max_p;
+    const int numP = m.plog_np[p] - 1;
+    const double logP = st.logP;
+    {
+        const bool lower = logP <= plog_par(m, 0, p, 0);
+        const bool upper = logP > plog_par(m, 0, p, numP);
+        int lo = 0, hi = 0;
+        if (upper) { lo = numP; hi = numP; }
+        for (int k = 0; k < mp - 1; ++k)
+            if (k < numP && logP > plog_par(m, 0, p, k) && logP <= plog_par(m, 0, p, k + 1)) {
+                lo = k;
+                hi = k + 1;
+            }
+        const double logk1 = plog_par(m, 1, p, lo) + st.logT * plog_par(m, 2, p, lo) -
+                             plog_par(m, 3, p, lo) * st.Tinv;
+        const double logk2 = plog_par(m, 1, p, hi) + st.logT * plog_par(m, 2, p, hi) -
+                             plog_par(m, 3, p, hi) * st.Tinv;
+        double kt = logk1;
+        if (!(lower || upper))
+            kt = (-logk1 + logk2) * (logP - plog_par(m, 0, p, lo)) /
+                     (plog_par(m, 0, p, hi) - plog_par(m, 0, p, lo)) +
+                 kt;
+        c.ksp(p) = exp(fmin(EXP_MAX, kt));
+    }
+    if (!JAC) return;
+    const double Tinv = st.Traw_inv;
+    int lo = 0, hi = numP;
+    for (int k = 0; k < mp - 1; ++k)
+        if (k <= numP && logP > plog_par(m, 0, p, k) && logP <= plog_par(m, 0, p, k + 1)) {
+            lo = k;
+            hi = k + 1;
+        }
+    double dkT;
+    if (logP > plog_par(m, 0, p, numP))
+        dkT = (plog_par(m, 2, p, hi) + plog_par(m, 3, p, hi) * Tinv) * Tinv;
+    else
+        dkT = (plog_par(m, 2, p, lo) + plog_par(m, 3, p, lo) * Tinv) * Tinv;
+    const bool inr = logP > plog_par(m, 0, p, 0) && logP <= plog_par(m, 0, p, numP);
+    if (inr)
+        dkT = dkT + Tinv * (logP - plog_par(m, 0, p, lo)) *
+                        (plog_par(m, 2, p, hi) - plog_par(m, 2, p, lo) +
+                         (plog_par(m, 3, p, hi) - plog_par(m, 3, p, lo)) * Tinv) /
+                        (plog_par(m, 0, p, hi) - plog_par(m, 0, p, lo));
+    double dkP = 0.0;
+    if (!c.conp && inr) {
+        const double logT = log(fmax(SMALL, st.T));
+        dkP = (plog_par(m, 1, p, hi) - plog_par(m, 1, p, lo) +
+               logT * (plog_par(m, 2, p, hi) - plog_par(m, 2, p, lo)) -
+               (plog_par(m, 3, p, hi) - plog_par(m, 3, p, lo)) * Tinv) /
+              (st.P * (plog_par(m, 0, p, hi) - plog_par(m, 0, p, lo)));
+    }
+    c.dkT(p) = dkT;
+    c.dkP(p) = dkP;
+}
+
+// Chebyshev polynomials by the recurrence T_p = 2 x T_{p-1} - T_{p-2}
+// (first = x: first kind, first = 2x: second kind)
+struct ChebPoly {
+    double x, pm2, pm1;
+    int p;
+    __device__ ChebPoly(double x_, double first) : x(x_), pm2(1.0), pm1(first), p(0) {}
+    __device__ double next() {
+        double v;
+        if (p == 0) v = 1.0;
+        else if (p == 1) v = pm1;
+        else {
+            v = 2.0 * x * pm1 - pm2;
+            pm2 = pm1;
+            pm1 = v;
+        }
+        ++p;
+        return v;
+    }
+};
+
+// Chebyshev rate constant (rate_subs.py:1904-2081) and its temperature /
+// pressure derivatives (create_jacobian.py:2823-2976, :1158-1300)
+template <bool JAC>
+__device__ void cheb_task(Ctx& c, const State& st, int q) {
+    const DevMech& m = c.m;
+    const int maxT = m.cheb_max_t, maxP = m.cheb_max_p;
+    const double Pmin = m.cheb_plim[2 * q], Pmax = m.cheb_plim[2 * q + 1];
+    const double Tmin = m.cheb_tlim[2 * q], Tmax = m.cheb_tlim[2 * q + 1];
+    const double* par = m.cheb_params + (size_t)q * maxT * maxP;
+    const double Pred = (-Pmax - Pmin + 2.0 * st.logP) / (Pmax - Pmin);
+    const int kidx = m.n_plog + q;
+    {
+        const double Tred = (-Tmax - Tmin + 2.0 * st.Tinv) / (Tmax - Tmin);
+        const int numP = m.cheb_np[q], numT = m.cheb_nt[q];
+        double kt = 0.0;
+        ChebPoly tp(Tred, Tred);
+        for (int t = 0; t < maxT; ++t) {
+            const double tt = tp.next();
+            double temp = 0.0;
+            ChebPoly pp(Pred, Pred);
+            for (int k = 0; k < maxP; ++k) {
+                const double pk = pp.next();
+                if (k < numP) temp = temp + pk * par[t * maxP + k];
+            }
+            if (t < numT) kt = kt + tt * temp;
+        }
+        c.ksp(kidx) = exp(LN10 * fmin(EXP_MAX, kt));
+    }
+    if (!JAC) return;
+    const double Tinv = st.Traw_inv;
+    const double Tred = (2 * Tinv - Tmax - Tmin) / (Tmax - Tmin);
+    const double Pred2 = (2 * st.logP - Pmax - Pmin) / (Pmax - Pmin);
+    double dkT, dkP = 0.0;
+    {
+        const int numP = m.cheb_np[q], numT = m.cheb_nt[q] - 1;
+        double dkf = 0;
+        ChebPoly tp(Tred, 2 * Tred);
+        for (int t = 0; t < maxT - 1; ++t) {
+            const double tt = tp.next();
+            double temp = 0;
+            ChebPoly pp(Pred2, Pred2);
+            for (int k = 0; k < maxP; ++k) {
+                const double pk = pp.next();
+                if (k < numP) temp = temp + pk * par[(t + 1) * maxP + k];
+            }
+            if (t < numT) dkf = dkf + (t + 1) * tt * temp;
+        }
+        dkT = -dkf * 2 * LN10 * Tinv * Tinv / (Tmax - Tmin);
+    }
+    if (!c.conp) {
+        const int numP = m.cheb_np[q] - 1, numT = m.cheb_nt[q];
+        double dkf = 0;
+        ChebPoly tp(Tred, Tred);
+        for (int t = 0; t < maxT; ++t) {
+            const double tt = tp.next();
+            double temp = 0;
+            ChebPoly pp(Pred2, 2 * Pred2);
+            for (int k = 0; k < maxP - 1; ++k) {
+                const double pk = pp.next();
+                if (k < numP) temp = temp + (k + 1) * pk * par[t * maxP + k + 1];
+            }
+            if (t < numT) dkf = dkf + tt * temp;
+        }
+        dkP = dkf * 2 * LN10 / (st.P * (Pmax - Pmin));
+    }
+    c.dkT(kidx) = dkT;
+    c.dkP(kidx) = dkP;
+}
+
+// tasks are dealt to the waves in contiguous chunks of the kind-ordered
+// list, so each wave sees as few code paths as possible
+template <bool JAC, int WPS>
+__device__ __forceinline__ void phase_pdep(Ctx& c, const State& st, int wave, int lane) {
+    const DevMech& m = c.m;
+    const int n1 = m.n_thd, n2 = n1 + m.n_plog, n3 = n2 + m.n_cheb;
+    const int per = (n3 + WPS - 1) / WPS;
+    const int end = min(n3, (wave + 1) * per);
+    for (int i = wave * per + lane; i < end; i += 64) {
+        if (i < n1) thd_task<JAC>(c, st, c.bt.thd_order[i]);
+        else if (i < n2) plog_task<JAC>(c, st, i - n1);
+        else cheb_task<JAC>(c, st, i - n2);
+    }
+}
+
+// ------------------------------------------------------------- phase 3
+// group flags of 64 consecutive reaction records (built by the runtime)
+constexpr int GF_CNT = 0xf;          // largest species count of the group
+constexpr int GF_REV = 1 << 4;       // a reversible reaction
+constexpr int GF_THD = 1 << 5;       // a third-body / falloff reaction
+constexpr int GF_SPECIAL = 1 << 6;   // a PLOG / Chebyshev reaction
+constexpr int GF_NS = 1 << 7;        // the bath gas among some reaction's species
+constexpr int GF_PAIRS = 1 << 8;     // third-body columns
+constexpr int GF_POW = 1 << 9;       // Arrhenius form A T^b
+
+// c^e for e >= 0 as the reference's product chain, branch-free up to e = 3
+__device__ __forceinline__ double powu(double v, int e) {
+    double r = e >= 1 ? v : 1.0;
+    r = e >= 2 ? r * v : r;
+    r = e >= 3 ? r * v : r;
+    for (int i = 3; i < e; ++i) r *= v;
+    return r;
+}
+
+// rate constant, reverse rate constant and the forward / reverse rates of
+// progress of one reaction (rate_subs.py:2803-2870, :1627-1785,
+// :1371-1507), concentrations from LDS
+struct Rop {
+    double kf, kr, rf, rr, ci;
+    double cv[MAXSP];
+};
+
+__device__ __forceinline__ Rop rates_of_progress(Ctx& c, const State& st, const Rec& x, int gf) {
+    Rop v;
+    const int cm = gf & GF_CNT;
+#pragma unroll
+    for (int q = 0; q < MAXSP; ++q) v.cv[q] = q < cm ? (double)c.conc(x.spec(q)) : 1.0;
+    // one exponential serves forms 2-4 (form 3 stores Ta = 0, form 4 b = 0)
+    const double ek = exp(fmin(EXP_MAX, x.A + st.logT * x.b + -1.0 * x.Ta * st.Tinv));
+    double kf = x.kind >= 2 ? ek : x.A;
+    if (gf & GF_POW) {
+        if (x.kind == 1) kf = x.A * powi(st.Tc, (int)x.b);
+    }
+    if (gf & GF_SPECIAL) {
+        if (x.kind >= 5) kf = c.ksp(x.kidx);
+    }
+    v.kf = kf;
+    double rf = kf;
+#pragma unroll
+    for (int q = 0; q < MAXSP; ++q)
+        if (q < cm) rf = rf * powu(v.cv[q], x.nur(q));
+    v.rf = rf;
+    v.kr = 0.0;
+    v.rr = 0.0;
+    if (gf & GF_REV) {
+        // equilibrium constant from the Gibbs terms (padding slots add 0)
+        double B_sum = 0.0;
+#pragma unroll
+        for (int q = 0; q < MAXSP; ++q) {
+            const int net_nu = x.nu(q);
+            if (q < cm && net_nu != 0) B_sum = B_sum + (double)net_nu * c.b(x.spec(q));
+        }
+        B_sum = exp(fmin(EXP_MAX, B_sum));
+        const double P_val = !(x.nsum > 0) ? RU / PA : PA / RU;
+        const double P_sum = powu(P_val, x.nsum < 0 ? -x.nsum : x.nsum);
+        const double kr = kf / (P_sum * B_sum);
+        double rr = kr;
+#pragma unroll
+        for (int q = 0; q < MAXSP; ++q)
+            if (q < cm) rr = rr * powu(v.cv[q], x.nup(q));
+        v.kr = x.rev ? kr : 0.0;
+        v.rr = x.rev ? rr : 0.0;
+    }
+    v.ci = 1.0;
+    if (gf & GF_THD) {
+        if (x.thd >= 0) v.ci = c.ci(x.thd);
+    }
+    return v;
+}
+
+// nu_k * rop into the production rates (get_spec_rates, rate_subs.py:1079-1167);
+// slots without a net coefficient add 0 to the reaction's own row
+__device__ __forceinline__ void scatter_wdot(Ctx& c, const Rec& x, double net, int gf) {
+    const int cm = gf & GF_CNT;
+#pragma unroll
+    for (int q = 0; q < MAXSP; ++q) {
+        if (q >= cm) break;
+        const int nu = x.nu(q);
+        lds_add(&c.wdot(nu != 0 ? x.spec(q) : x.home), (double)nu * net);
+    }
+}
+
+// all per-reaction Jacobian contributions of one reaction: dRopidnj(+_ns)
+// (create_jacobian.py:4169-4484), dRopi*_dT(+_ns) (:2978-2992 and the _ns
+// kernels), dRopi*_dE(+_ns) (:1040-1080, :1158-1300) and the third-body
+// parts through the per-entry factors of phase 2, scattered into LDS.
+// Slots that contribute nothing add 0 to an entry of the reaction's own
+// row instead of branching.
+__device__ __forceinline__ void reaction_jacobian(Ctx& c, const State& st, const Rec& x,
+                                                  const Rop& v, double ropi, int gf) {
+    const int ns = c.ns, NN = c.NN;
+    const int cm = gf & GF_CNT;
+    const int conp = c.conp;
+    const double Tinv = st.Traw_inv;
+    const double kr_i = v.kr;
+    lds_double* J = c.J();
+    // rows: species with a net coefficient other than the bath gas;
+    // columns: species other than the bath gas
+    int roff[MAXSP], coff[MAXSP];
+    double nuk[MAXSP];
+    const int hrow = __umul24(x.home + 2, NN), hcol = x.home + 2;
+#pragma unroll
+    for (int q = 0; q < MAXSP; ++q) {
+        const int spec = x.spec(q), nu = x.nu(q);
+        const bool in = q < x.cnt && spec != ns;
+        coff[q] = in ? spec + 2 : hcol;
+        const bool rowok = in && nu != 0;
+        roff[q] = rowok ? __umul24(spec + 2, NN) : hrow;
+        nuk[q] = rowok ? (double)nu : 0.0;
+    }
+    // d(prod_i c_i^nu_i)/dc_j = nu_j c_j^(nu_j-1) prod_{i != j} c_i^nu_i
+    // with the concentrations floored at 1e-300: suffix products first,
+    // then one forward sweep that scatters column j as soon as it is known
+    double sfx_f[MAXSP], sfx_r[MAXSP];
+    {
+        double af = 1.0, ar = 1.0;
+#pragma unroll
+        for (int q = MAXSP - 1; q >= 0; --q) {
+            sfx_f[q] = af;
+            sfx_r[q] = ar;
+            if (q < cm) {
+                const double cg = fmax(SMALL, v.cv[q]);
+                af *= powu(cg, x.nur(q));
+                ar *= powu(cg, x.nup(q));
+            }
+        }
+    }
+    double Sf_l = 0.0, Sr_l = 0.0;
+    {
+        double pf = 1.0, pr = 1.0;
+#pragma unroll
+        for (int j = 0; j < MAXSP; ++j) {
+            if (j >= cm) break;
+            const double cg = fmax(SMALL, v.cv[j]);
+            const int a = x.nur(j), b = x.nup(j);
+            const double df = a ? a * powu(cg, a > 0 ? a - 1 : 0) : 0.0;
+            const double dr = b ? b * powu(cg, b > 0 ? b - 1 : 0) : 0.0;
+            const double Sf = pf * df * sfx_f[j];
+            const double Sr = pr * dr * sfx_r[j];
+            if (j == x.cnt - 1) { Sf_l = Sf; Sr_l = Sr; }
+            pf *= powu(cg, a);
+            pr *= powu(cg, b);
+            const bool colok = j < x.cnt && x.spec(j) != ns;
+            const double d = colok ? (v.kf * Sf + -1.0 * kr_i * Sr) * v.ci : 0.0;
+#pragma unroll
+            for (int k = 0; k < MAXSP; ++k) {
+                if (k >= cm) break;
+                lds_add(J + roff[k] + coff[j], nuk[k] * d);
+            }
+        }
+    }
+    // temperature and extra-variable derivatives of the rate of progress
+    double dkT = (x.b + x.Ta * Tinv) * Tinv, dkP = 0.0;
+    if (gf & GF_SPECIAL) {
+        if (x.kind > 4) { dkT = c.dkT(x.kidx); dkP = c.dkP(x.kidx); }
+    }
+    double dT, dE = 0.0;
+    {
+        double dd = v.rf * dkT;
+        if (gf & GF_REV) {
+            double dB = 0.0;
+#pragma unroll
+            for (int q = 0; q < MAXSP; ++q)
+                if (q < cm) dB = dB + (double)x.nu(q) * c.db(x.spec(q));
+            if (x.rev) dd = dd + -1.0 * v.rr * (dkT + -1.0 * dB);
+        }
+        dT = dd * v.ci * st.V;
+    }
+    if (conp) {
+        int nu_f = -1, nu_r = -1;
+#pragma unroll
+        for (int q = 0; q < MAXSP; ++q)
+            if (q < cm) { nu_f += x.nur(q); nu_r += x.nup(q); }
+        double dd = -1.0 * nu_f * v.rf;
+        if (x.rev) dd = dd + nu_r * v.rr;
+        if (x.thd >= 0) dd = dd * v.ci;
+        dE = dd;
+    } else if (gf & GF_SPECIAL) {
+        if (x.kind > 4) dE = (v.rf - v.rr) * dkP * st.V;
+    }
+    double u = 0.0;
+    if (gf & GF_NS) {
+        // bath gas among the reaction's species (always its last slot): the
+        // _ns kernels, d/dn_ns = -d/dn_j for every species column j
+        int last = 0;
+#pragma unroll
+        for (int q = 0; q < MAXSP; ++q)
+            if (q == x.cnt - 1) last = x.spec(q);
+        if (last == ns) {
+            dT += (Sr_l * kr_i + -1.0 * Sf_l * v.kf) * st.V * v.ci * st.P / (RU * st.T * st.T);
+            const double fac = conp ? st.P * st.rt_inv : st.V * st.rt_inv;
+            dE += (Sf_l * v.kf - Sr_l * kr_i) * v.ci * fac;
+            u = (kr_i * Sr_l + -1.0 * v.kf * Sf_l) * v.ci;
+        }
+    }
+    if (gf & GF_THD) {
+        if (x.thd >= 0) {
+            // third-body columns: nu_k * d(ci)/dn_j * rop, accumulated per
+            // (efficiency set, row species) pair and spread in phase 4
+            const int t = x.thd;
+            const double trop = ropi * c.nfac(t);
+            u += c.bt.effns[t] * trop;
+            dT += c.dTf(t) * ropi;
+            dE += c.dEf(t) * ropi;
+            if (gf & GF_PAIRS) {
+                const int4 p0 = ((const int4*)c.bt.thd_pairs)[2 * t];
+                const int2 p1 = ((const int2*)c.bt.thd_pairs)[4 * t + 2];
+                const int pp[MAXSP] = {p0.x, p0.y, p0.z, p0.w, p1.x, p1.y};
+#pragma unroll
+                for (int q = 0; q < MAXSP; ++q)
+                    if (q < cm && pp[q] >= 0) lds_add(&c.Cp(pp[q]), (double)x.nu(q) * trop);
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < MAXSP; ++k) {
+        if (k >= cm) break;
+        lds_add(J + roff[k], nuk[k] * dT);
+        lds_add(J + roff[k] + 1, nuk[k] * dE);
+    }
+    if (gf & (GF_NS | GF_THD)) {
+#pragma unroll
+        for (int k = 0; k < MAXSP; ++k) {
+            if (k >= cm) break;
+            const int spec = x.spec(k);
+            const bool rowok = k < x.cnt && spec != ns && x.nu(k) != 0;
+            lds_add(&c.U(rowok ? spec : x.home), nuk[k] * u);
+        }
+    }
+}
+
+// ------------------------------------------------------------- kernels
+
+struct Layout {
+    size_t sN, sI;  // stride between states, between entries of a state
+};
+
+// XCD-aware state of a workgroup: workgroup g runs on XCD g % 8, so give
+// each XCD a contiguous range of states; neighbouring states then run on
+// one XCD at the same time and their state-fastest (F order) reads and
+// writes meet in that XCD's L2.  The grid is 8 * ceil(n / 8) workgroups.
+__device__ __forceinline__ long state_of_block(long g, long n) {
+    const long per = (n + 7) / 8;
+    return (g % 8) * per + g / 8;
+}
+
+__device__ __forceinline__ State load_state(int conp, double param, double x0, double x1) {
+    State st;
+    st.T = x0;
+    if (conp) { st.P = param; st.V = x1; }
+    else { st.P = x1; st.V = param; }
+    st.Tc = clampT(st.T);
+    st.logT = log(st.Tc);
+    st.Tinv = 1.0 / st.Tc;
+    st.logP = log(fmax(SMALL, st.P));
+    st.Traw_inv = 1.0 / st.T;
+    st.rt_inv = 1.0 / (RU * st.T);
+    st.V_inv = 1.0 / fmax(V_MIN, st.V);
+    return st;
+}
+
+template <int WPS>
+__global__ void __launch_bounds__(64 * WPS)
+rates_block_kernel(const BlockArgs* __restrict__ A, int conp, long n,
+                   const double* __restrict__ param, const double* __restrict__ phi,
+                   double* __restrict__ dphi, Layout L) {
+    constexpr int NT = 64 * WPS;
+    extern __shared__ double smem[];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const long s = state_of_block(blockIdx.x, n);
+    if (s >= n) return;
+    Ctx c = make_ctx(A, smem, conp);
+    const int Ns = c.Ns, Nr = c.m.Nr;
+    if (tid == 0) { c.conc(Ns) = 1.0; c.b(Ns) = 0.0; }
+    const double* ps = phi + s * L.sN;
+    auto ph = [&](int i) { return ps[i * L.sI]; };
+    const State st = load_state(conp, param[s], ph(0), ph(1));
+    phase_species<false, NT>(c, st, tid, ph);
+    __syncthreads();
+    bath_gas(c, st, lane);
+    phase_pdep<false, WPS>(c, st, wave, lane);
+    __syncthreads();
+    for (int pass = 0; pass * NT < Nr; ++pass) {
+        const int r = pass * NT + tid;
+        const int gf = sld(c.bt.grp_flags, pass * WPS + wave);
+        if (r < Nr) {
+            const Rec x = load_rec(c.bt, r);
+            const Rop v = rates_of_progress(c, st, x, gf);
+            scatter_wdot(c, x, (v.rf - v.rr) * v.ci, gf);
+        }
+    }
+    __syncthreads();
+    // dphi (rate_subs.py:709-1076); the three sums in the reference's
+    // sequential order, one lane each
+    if (wave == 0) {
+        const double V_val = fmax(V_MIN, st.V);
+        double acc = 0.0;
+        if (lane < 3) {
+            for (int k = 0; k < Ns; ++k) {
+                const double w = c.wdot(k);
+                const double mwf = k < Ns - 1 ? 1.0 + -1.0 * sld(c.m.mw_post, k) : 0.0;
+                const double a = lane == 0 ? (double)c.conc(k) : (lane == 1 ? (double)c.energy(k) : mwf);
+                const double bb = lane == 0 ? (double)c.heat(k) : w;
+                acc = __dadd_rn(acc, __dmul_rn(a, bb));
+            }
+        }
+        const double lower = __shfl(acc, 0), upper = __shfl(acc, 1), dE = __shfl(acc, 2);
+        const double Tdot = 0.0 + -1.0 * upper / lower;
+        double Edot;
+        if (conp) {
+            Edot = V_val * Tdot / st.Tc;
+            Edot = Edot + V_val * dE * st.Tc * RU / st.P;
+        } else {
+            Edot = st.P * Tdot / st.Tc;
+            Edot = Edot + st.Tc * RU * dE;
+        }
+        double* out = dphi + s * L.sN;
+        for (int k = lane; k < Ns + 1; k += 64) {
+            double val;
+            if (k == 0) val = Tdot;
+            else if (k == 1) val = Edot;
+            else val = V_val * c.wdot(k - 2);
+            out[k * L.sI] = val;
+        }
+    }
+}
+
+template <int WPS>
+__global__ void __launch_bounds__(64 * WPS)
+jacobian_block_kernel(const BlockArgs* __restrict__ A, int conp, long n,
+                      const double* __restrict__ param, const double* __restrict__ phi,
+                      double* __restrict__ jac, Layout L, int out_C) {
+    constexpr int NT = 64 * WPS;
+    extern __shared__ double smem[];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const long s = state_of_block(blockIdx.x, n);
+    if (s >= n) return;
+    Ctx c = make_ctx(A, smem, conp);
+    const int Ns = c.Ns, NN = c.NN, ns = c.ns, Nr = c.m.Nr, nout = c.m.jac_nnz;
+    lds_double* J = c.J();
+    if (tid == 0) { c.conc(Ns) = 1.0; c.b(Ns) = 0.0; c.db(Ns) = 0.0; }
+    for (int k = tid; k < Ns - 1; k += NT) c.mw(k) = c.m.mw_post[k];
+    const double* ps = phi + s * L.sN;
+    auto ph = [&](int i) { return ps[i * L.sI]; };
+    const State st = load_state(conp, param[s], ph(0), ph(1));
+    {
+        typedef double v2d __attribute__((ext_vector_type(2)));
+        typedef __attribute__((address_space(3))) v2d lds_v2d;
+        lds_v2d* J2 = (lds_v2d*)J;
+        const int half = (NN * NN) / 2;
+        for (int e = tid; e < half; e += NT) J2[e] = v2d{0.0, 0.0};
+        if (tid == 0 && (NN * NN) % 2) J[NN * NN - 1] = 0.0;
+        for (int p = tid; p < c.bt.n_pairs; p += NT) c.Cp(p) = 0.0;
+    }
+    phase_species<true, NT>(c, st, tid, ph);
+    __syncthreads();
+    bath_gas(c, st, lane);
+    phase_pdep<true, WPS>(c, st, wave, lane);
+    __syncthreads();
+    // phase 3: reactions
+    for (int pass = 0; pass * NT < Nr; ++pass) {
+        const int r = pass * NT + tid;
+        const int gf = sld(c.bt.grp_flags, pass * WPS + wave);
+        if (r < Nr) {
+            const Rec x = load_rec(c.bt, r);
+            const Rop v = rates_of_progress(c, st, x, gf);
+            const double ropi = v.rf - v.rr;
+            scatter_wdot(c, x, ropi * v.ci, gf);
+            reaction_jacobian(c, st, x, v, ropi, gf);
+        }
+    }
+    __syncthreads();
+    // phase 4: third-body columns d(ci)/dn_j (create_jacobian.py
+    // dci_thd_dnj / dci_lind / dci_sri / dci_troe): every non-zero
+    // (pair, column) of the host's list adds coefficient * efficiency
+    {
+        const int4* cb = (const int4*)c.bt.combo;
+        for (int i = tid; i < c.bt.n_combo; i += NT) {
+            const int4 e = cb[i];
+            lds_add(J + e.y, c.Cp(e.x) * __hiloint2double(e.w, e.z));
+        }
+    }
+    // temperature rate and total heat capacity (every wave)
+    double heat_tot, Tdot;
+    {
+        double lo = 0.0, up = 0.0;
+        for (int k = lane; k < Ns; k += 64) {
+            lo += c.conc(k) * c.heat(k);
+            up += c.energy(k) * c.wdot(k);
+        }
+        heat_tot = wave_sum(lo);
+        Tdot = 0.0 + -1.0 * wave_sum(up) / heat_tot;
+    }
+    __syncthreads();
+    // phase 5: rows 0 and 1 of the species columns (dTdot_dnj, dEdot_dnj,
+    // create_jacobian.py:3266-3484): lanes over columns, waves over blocks
+    // of rows; the bath-gas sums U join every species column of their row
+    const double T = st.T, Tinv = st.Traw_inv;
+    const double fixed = conp ? st.P : st.V;
+    const double extra = conp ? st.V : st.P;
+    const double e_ns = c.energy(ns), h_ns = c.heat(ns);
+    const double inv_vh = 1.0 / (st.V * heat_tot);
+    {
+        const int per = (Ns - 1 + WPS - 1) / WPS;
+        const int k0 = wave * per, k1 = min(Ns - 1, k0 + per);
+        for (int j = lane; j < Ns - 1; j += 64) {
+            double sum = 0.0, sum2 = 0.0;
+            for (int k = k0; k < k1; ++k) {
+                const double jv = J[(k + 2) * NN + j + 2] + c.U(k);
+                const double mw = c.mw(k);
+                sum = sum + (c.energy(k) + -1.0 * e_ns * mw) * jv;
+                sum2 = sum2 + (1.0 + -1.0 * mw) * jv;
+            }
+            double j0 = -1.0 * sum * inv_vh;
+            if (wave == 0) j0 = j0 + -1.0 * Tdot * (c.heat(j) + -1.0 * h_ns) * inv_vh;
+            const double j1 = T * RU * sum2 / fixed + extra * j0 * Tinv;
+            if (WPS == 1) {
+                J[j + 2] = j0;
+                J[NN + j + 2] = j1;
+            } else {
+                lds_add(J + j + 2, j0);
+                lds_add(J + NN + j + 2, j1);
+            }
+        }
+    }
+    if (wave == WPS - 1) {
+        // columns 0 and 1 of rows 0 and 1 (create_jacobian.py:1595-2114)
+        const double Vinv = 1.0 / st.V;
+        double rate_sum = 0.0, sumT = 0.0, dTsumE = 0.0, specsum = 0.0, sumE = 0.0,
+               sumEE = 0.0, dTsum = 0.0;
+        for (int i = lane; i < Ns - 1; i += 64) {
+            const double j0 = J[(i + 2) * NN];
+            const double j1 = J[(i + 2) * NN + 1];
+            const double mw = c.mw(i);
+            const double w = c.wdot(i);
+            rate_sum += Vinv * j0 * (-1.0 * c.energy(i) + e_ns * mw) +
+                        w * (-1.0 * c.heat(i) + mw * h_ns);
+            dTsum += (h_ns * Tinv + -1.0 * c.dheat(i)) * c.conc(i);
+            if (conp) {
+                sumT += (1.0 + -1.0 * mw) * (Vinv * j0 + Tinv * w);
+                dTsumE += (c.heat(i) + -1.0 * h_ns) * c.conc(i);
+                specsum += (c.energy(i) + -1.0 * e_ns * mw) * (j1 + -1.0 * w);
+            } else {
+                sumT += (1.0 - mw) * (j0 * (T / st.V) + w);
+                sumE += (c.energy(i) - e_ns * mw) * j1;
+            }
+            sumEE += (1 - mw) * j1;
+        }
+        rate_sum = wave_sum(rate_sum);
+        dTsum = wave_sum(dTsum) + (h_ns * Tinv + -1.0 * c.dheat(ns)) * c.conc(ns);
+        sumT = wave_sum(sumT);
+        sumEE = wave_sum(sumEE);
+        if (conp) {
+            dTsumE = wave_sum(dTsumE);
+            specsum = wave_sum(specsum);
+        } else {
+            sumE = wave_sum(sumE);
+        }
+        if (lane == 0) {
+            const double j00 = (Tdot * dTsum + rate_sum) / heat_tot;
+            J[0] = j00;
+            double j10;
+            if (conp) {
+                j10 = st.V * Tinv * (j00 + -1.0 * Tinv * Tdot);
+                j10 = j10 + RU * T * st.V * sumT / st.P;
+            } else {
+                j10 = st.P * (j00 - Tdot * Tinv) * Tinv;
+                j10 = j10 + RU * sumT;
+            }
+            J[NN] = j10;
+            double j01;
+            if (conp) {
+                j01 = (Tdot * dTsumE + -1.0 * specsum) * (1.0 / (heat_tot * st.V));
+            } else {
+                const double spec_inv = 1.0 / heat_tot;
+                j01 = -(Tdot * h_ns / (RU * T)) * spec_inv;
+                j01 = j01 - (sumE * Vinv) * spec_inv;
+            }
+            J[1] = j01;
+            const double var = conp ? st.V : st.P;
+            const double par = conp ? st.P : st.V;
+            const double j11 = (var * j01 + Tdot) / T;
+            J[NN + 1] = j11 + RU * T * sumEE / par;
+        }
+    }
+    __syncthreads();
+    // phase 6: stored entries of this state, four per thread in flight
+    {
+        const int2* op = (const int2*)c.bt.out_pos;
+        constexpr int UNR = 4;
+        for (int e0 = tid; e0 < nout; e0 += NT * UNR) {
+            int2 p[UNR];
+#pragma unroll
+            for (int u = 0; u < UNR; ++u) {
+                const int e = e0 + u * NT;
+                p[u] = e < nout ? op[e] : make_int2(0, -1);
+            }
+            double val[UNR];
+#pragma unroll
+            for (int u = 0; u < UNR; ++u) {
+                val[u] = J[p[u].x];
+                if (p[u].y >= 0) val[u] += c.U(p[u].y);
+            }
+#pragma unroll
+            for (int u = 0; u < UNR; ++u) {
+                const int e = e0 + u * NT;
+                if (e < nout) {
+                    if (out_C) jac[s * (size_t)nout + e] = val[u];
+                    else jac[(size_t)e * n + s] = val[u];
+                }
+            }
+        }
+    }
+}
+
+}  // namespace pjb
+
+namespace pjk_host {
+
+BlockLds block_lds(const DevMech& m, bool jac, int bt_pairs) {
+    BlockLds L{};
+    int o = 0;
+    const int Ns = m.Ns;
+    auto take = [&](int cnt) {
+        int at = o;
+        o += cnt > 0 ? cnt : 1;
+        return at;
+    };
+    L.conc = take(Ns + 1);
+    L.nmol = take(Ns);
+    L.b = take(Ns + 1);
+    L.wdot = take(Ns + 1);
+    L.energy = take(Ns);
+    L.heat = take(Ns);
+    L.ci = take(m.n_thd);
+    L.ksp = take(m.n_plog + m.n_cheb);
+    if (jac) {
+        L.db = take(Ns + 1);
+        L.dheat = take(Ns);
+        L.U = take(Ns);
+        L.nfac = take(m.n_thd);
+        L.dTf = take(m.n_thd);
+        L.dEf = take(m.n_thd);
+        L.Cp = take(bt_pairs);
+        L.mw = take(Ns - 1);
+        L.dkT = take(m.n_plog + m.n_cheb);
+        L.dkP = take(m.n_plog + m.n_cheb);
+        o = (o + 1) & ~1;  // 16-byte alignment of the accumulator
+        L.jac = take(m.NN * m.NN);
+    }
+    L.total = o;
+    return L;
+}
+
+size_t block_lds_bytes(const DevMech& m, bool jac, int n_pairs) {
+    return (size_t)block_lds(m, jac, n_pairs).total * sizeof(double);
+}
+
+template <class K>
+static hipError_t set_lds(K kernel, size_t lds) {
+    if (lds > 64 * 1024)
+        return hipFuncSetAttribute((const void*)kernel,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    return hipSuccess;
+}
+
+hipError_t launch_rates_block(const DevMech& m, const BlockArgs* args, const BlockLds& BL, int wps,
+                              int conp, char order, long n, const double* param,
+                              const double* phi, double* dphi, hipStream_t stream) {
+    pjb::Layout L = order == 'F' ? pjb::Layout{1, (size_t)n} : pjb::Layout{(size_t)m.NN, 1};
+    size_t lds = (size_t)BL.total * sizeof(double);
+    const long grid = 8 * ((n + 7) / 8);
+#define PJB_RATES(W)                                                                      \
+    case W: {                                                                             \
+        hipError_t e = set_lds(pjb::rates_block_kernel<W>, lds);                         \
+        if (e != hipSuccess) return e;                                                    \
+        hipLaunchKernelGGL(pjb::rates_block_kernel<W>, dim3(grid), dim3(64 * W), lds,     \
+                           stream, args, conp, n, param, phi, dphi, L);              \
+        break;                                                                            \
+    }
+    switch (wps) {
+        PJB_RATES(1)
+        PJB_RATES(2)
+        PJB_RATES(4)
+        PJB_RATES(8)
+        default: return hipErrorInvalidValue;
+    }
+#undef PJB_RATES
+    return hipGetLastError();
+}
+
+hipError_t launch_jacobian_block(const DevMech& m, const BlockArgs* args, const BlockLds& BL,
+                                 int wps, int conp, char order, long n, const double* param,
+                                 const double* phi, double* jac, hipStream_t stream) {
+    pjb::Layout L = order == 'F' ? pjb::Layout{1, (size_t)n} : pjb::Layout{(size_t)m.NN, 1};
+    size_t lds = (size_t)BL.total * sizeof(double);
+    const long grid = 8 * ((n + 7) / 8);
+    const int out_C = order == 'C';
+#define PJB_JAC(W)                                                                        \
+    case W: {                                                                             \
+        hipError_t e = set_lds(pjb::jacobian_block_kernel<W>, lds);                      \
+        if (e != hipSuccess) return e;                                                    \
+        hipLaunchKernelGGL(pjb::jacobian_block_kernel<W>, dim3(grid), dim3(64 * W), lds,  \
+                           stream, args, conp, n, param, phi, jac, L, out_C);        \
+        break;                                                                            \
+    }
+    switch (wps) {
+        PJB_JAC(1)
+        PJB_JAC(2)
+        PJB_JAC(4)
+        PJB_JAC(8)
+        default: return hipErrorInvalidValue;
+    }
+#undef PJB_JAC
+    return hipGetLastError();
+}
+
+}  // namespace pjk_host

@@ -65,7 +65,14 @@ struct pyjac_handle {
     int n_cu = 256;
     DevMech m{};
     pjk_host::WaveTables wt{};
+    pjk_host::BlockTables bt{};
     bool wave_ok_rates = false, wave_ok_jac = false;
+    bool block_ok_rates = false, block_ok_jac = false;
+    // block kernels: device copies of their arguments, LDS layouts and the
+    // tuned number of waves per state (0: default)
+    pjk_host::BlockArgs* d_args[2] = {nullptr, nullptr};
+    pjk_host::BlockLds lds[2]{};
+    int wps[2] = {0, 0};
     char* d_blob = nullptr;
     // workspace of the persistent kernels
     double* d_ws = nullptr;
@@ -125,10 +132,33 @@ int wave_grid(const pyjac_handle* h, size_t lds_bytes, long n) {
     return (int)std::max(8L, G / 8 * 8);
 }
 
+// Launch shape of the block-per-state kernels.  Variants: 0 automatic,
+// 1 lane-per-state, 2 wave-per-state, 10 + W block-per-state with W waves
+// per state (W = 1, 2, 4, 8).
+constexpr size_t LDS_PER_CU = 160 * 1024;
+
+int block_wps(const pyjac_handle* h, int kernel) {
+    int v = h->variant[kernel];
+    if (v > 10) return v - 10;
+    if (h->wps[kernel]) return h->wps[kernel];
+    return 1;
+}
+
+bool use_block(const pyjac_handle* h, int kernel) {
+    int v = h->variant[kernel];
+    bool ok = kernel == 0 ? h->block_ok_rates : h->block_ok_jac;
+    return ok && (v == 0 || v > 10);
+}
+
 int run_rates(pyjac_handle* h, long n, const double* param, const double* phi, double* dphi,
               hipStream_t s) {
     if (n <= 0) return PYJAC_OK;
-    if (h->wave_ok_rates && h->variant[0] != 1) {
+    if (use_block(h, 0)) {
+        HIP_TRY(pjk_host::launch_rates_block(h->m, h->d_args[0], h->lds[0], block_wps(h, 0),
+                                             h->conp, h->order, n, param, phi, dphi, s));
+        return PYJAC_OK;
+    }
+    if (h->wave_ok_rates && h->variant[0] == 2) {
         size_t lds = pjk_host::wave_lds_bytes(h->m, false, 0);
         HIP_TRY(pjk_host::launch_rates_wave(h->m, h->conp, h->order, n, param, phi, dphi, h->wt,
                                             wave_grid(h, lds, n), s));
@@ -146,7 +176,12 @@ int run_rates(pyjac_handle* h, long n, const double* param, const double* phi, d
 int run_jac(pyjac_handle* h, long n, const double* param, const double* phi, double* jac,
             hipStream_t s) {
     if (n <= 0) return PYJAC_OK;
-    if (h->wave_ok_jac && h->variant[1] != 1) {
+    if (use_block(h, 1)) {
+        HIP_TRY(pjk_host::launch_jacobian_block(h->m, h->d_args[1], h->lds[1], block_wps(h, 1),
+                                                h->conp, h->order, n, param, phi, jac, s));
+        return PYJAC_OK;
+    }
+    if (h->wave_ok_jac && h->variant[1] == 2) {
         size_t lds = pjk_host::wave_lds_bytes(h->m, true, h->m.NN * h->m.NN);
         HIP_TRY(pjk_host::launch_jacobian_wave(h->m, h->conp, h->order, n, param, phi, jac, h->wt,
                                                wave_grid(h, lds, n), s));
@@ -294,6 +329,115 @@ int pyjac_hip_create(const pyjac_mech_tables* mt, int conp, char order, int jac_
         }
     }
 
+    // block-kernel tables: reaction records sorted by structure (third body,
+    // pressure-dependent kind, reversibility, bath gas, species count) so
+    // the lanes of a wave follow the same branches
+    std::vector<int> perm(Nr), pos_of(Nr);
+    for (int r = 0; r < Nr; ++r) perm[r] = r;
+    auto rkey = [&](int r) {
+        int cnt = mt->rxn_off[r + 1] - mt->rxn_off[r];
+        int has_ns = cnt > 0 && mt->rxn_spec[mt->rxn_off[r + 1] - 1] == Ns - 1;
+        int kind = rx_i[r].x & 0xff;
+        return ((mt->thd_mask[r] >= 0) << 12) | ((kind >= 5) << 11) |
+               ((mt->fall_mask[r] >= 0) << 10) | ((mt->rev_mask[r] >= 0) << 9) |
+               (has_ns << 8) | (kind << 4) | cnt;
+    };
+    std::stable_sort(perm.begin(), perm.end(), [&](int a, int b) { return rkey(a) < rkey(b); });
+    std::vector<I4> rec_i(Nr);
+    std::vector<D4> rec_d(Nr);
+    std::vector<int> rec_s((size_t)8 * Nr);
+    for (int q = 0; q < Nr; ++q) {
+        int r = perm[q];
+        pos_of[r] = q;
+        rec_i[q] = rx_i[r];
+        rec_i[q].y = mt->rev_mask[r] >= 0 ? 1 : 0;
+        rec_d[q] = rx_d[r];
+        for (int k = 0; k < 8; ++k) rec_s[(size_t)8 * q + k] = rx_s[(size_t)8 * r + k];
+        // slot 6: a species with a Jacobian row (net coefficient, not the
+        // bath gas); zero-valued contributions of the reaction go there
+        int home = 0;
+        for (int idx = mt->rxn_off[r + 1] - 1; idx >= mt->rxn_off[r]; --idx)
+            if (mt->rxn_spec[idx] != Ns - 1 && mt->nu_prod[idx] != mt->nu_reac[idx])
+                home = mt->rxn_spec[idx];
+        rec_s[(size_t)8 * q + 6] = home;
+        rec_s[(size_t)8 * q + 7] = 0;
+    }
+    // third-body tables: kind-ordered entry list (plain, Lindemann, SRI,
+    // Troe), efficiency of the bath gas in d/dT and d/dE, d(ci)/dn_j through
+    // the bath gas, and the third-body columns as coefficient pairs
+    // (distinct efficiency set, row species) with their non-zero columns
+    const int nt_ = mt->n_thd;
+    std::vector<int> thd_order, thd_pairs((size_t)8 * std::max(nt_, 1), -1);
+    std::vector<double> thd_mod(std::max(nt_, 1), 1.0), effns(std::max(nt_, 1), 0.0);
+    for (int cls = 0; cls < 4; ++cls)
+        for (int t = 0; t < nt_; ++t) {
+            int f = mt->fall_mask[mt->thd_map[t]];
+            int c = f < 0 ? 0 : (mt->fall_blend[f] == 1 ? 1 : (mt->fall_blend[f] == 3 ? 2 : 3));
+            if (c == cls) thd_order.push_back(t);
+        }
+    std::vector<std::vector<double>> eff_sets;
+    std::vector<std::pair<int, int>> pairs;  // (efficiency set, species)
+    for (int t = 0; t < nt_; ++t) {
+        const int tt = mt->thd_type[t], e0 = mt->thd_off[t], e1 = mt->thd_off[t + 1];
+        const bool last_ns = e1 > e0 && mt->thd_spec[e1 - 1] == Ns - 1;
+        if (tt == 2) thd_mod[t] = last_ns ? 1.0 : 0.0;
+        if (tt == 1 && last_ns) thd_mod[t] = mt->thd_eff[e1 - 1];
+        if (last_ns) effns[t] = tt == 2 ? -1.0 : (tt == 1 ? 1.0 + -1.0 * mt->thd_eff[e1 - 1] : 0.0);
+        std::vector<double> eff(Ns - 1, 0.0);
+        bool any = false;
+        for (int e = e0; e < e1; ++e) {
+            int j = mt->thd_spec[e];
+            if (j == Ns - 1) continue;
+            double d = tt == 2 ? 1.0 : (tt == 1 ? mt->thd_eff[e] + -1.0 : 0.0);
+            eff[j] = d;
+            any = any || d != 0.0;
+        }
+        if (!any) continue;
+        int g = (int)(std::find(eff_sets.begin(), eff_sets.end(), eff) - eff_sets.begin());
+        if (g == (int)eff_sets.size()) eff_sets.push_back(eff);
+        const int r = mt->thd_map[t];
+        for (int q = 0; q < pjk_host::WAVE_MAXSP; ++q) {
+            int idx = mt->rxn_off[r] + q;
+            if (idx >= mt->rxn_off[r + 1]) break;
+            int spec = mt->rxn_spec[idx];
+            if (spec == Ns - 1 || mt->nu_prod[idx] == mt->nu_reac[idx]) continue;
+            std::pair<int, int> key(g, spec);
+            int p = (int)(std::find(pairs.begin(), pairs.end(), key) - pairs.begin());
+            if (p == (int)pairs.size()) pairs.push_back(key);
+            thd_pairs[(size_t)8 * t + q] = p;
+        }
+    }
+    // flags of each group of 64 consecutive records (see pyjac_block.hip)
+    std::vector<int> grp_flags((Nr + 63) / 64, 0);
+    for (int q = 0; q < Nr; ++q) {
+        const int r = perm[q];
+        const int cnt = mt->rxn_off[r + 1] - mt->rxn_off[r];
+        const int kind = rx_i[r].x & 0xff;
+        int& f = grp_flags[q / 64];
+        f = (f & ~0xf) | std::max(f & 0xf, std::min(cnt, pjk_host::WAVE_MAXSP));
+        if (mt->rev_mask[r] >= 0) f |= 1 << 4;
+        if (mt->thd_mask[r] >= 0) f |= 1 << 5;
+        if (kind >= 5) f |= 1 << 6;
+        if (cnt > 0 && mt->rxn_spec[mt->rxn_off[r + 1] - 1] == Ns - 1) f |= 1 << 7;
+        if (kind == 1) f |= 1 << 9;
+    }
+    for (int t = 0; t < nt_; ++t)
+        for (int q = 0; q < 8; ++q)
+            if (thd_pairs[(size_t)8 * t + q] >= 0) grp_flags[pos_of[mt->thd_map[t]] / 64] |= 1 << 8;
+    struct Combo { int p, dst; double e; };
+    std::vector<Combo> combos;
+    for (size_t p = 0; p < pairs.size(); ++p)
+        for (int j = 0; j < Ns - 1; ++j) {
+            double e = eff_sets[pairs[p].first][j];
+            if (e != 0.0) combos.push_back(Combo{(int)p, (pairs[p].second + 2) * NN + j + 2, e});
+        }
+    std::vector<int> out_pos((size_t)2 * jac_pos.size());
+    for (size_t e = 0; e < jac_pos.size(); ++e) {
+        int p = jac_pos[e], row = p / NN, col = p % NN;
+        out_pos[2 * e] = p;
+        out_pos[2 * e + 1] = (row >= 2 && col >= 2) ? row - 2 : -1;
+    }
+
     Packer P;
     DevMech& m = h->m;
     m.Ns = Ns; m.Nr = Nr; m.NN = NN; m.ns = Ns - 1;
@@ -368,6 +512,16 @@ int pyjac_hip_create(const pyjac_mech_tables* mt, int conp, char order, int jac_
     size_t off_rx_i = P.add(rx_i.data(), rx_i.size());
     size_t off_rx_d = P.add(rx_d.data(), rx_d.size());
     size_t off_rx_s = P.add(rx_s.data(), rx_s.size());
+    size_t off_rec_i = P.add(rec_i.data(), rec_i.size());
+    size_t off_rec_d = P.add(rec_d.data(), rec_d.size());
+    size_t off_rec_s = P.add(rec_s.data(), rec_s.size());
+    size_t off_grp_flags = P.add(grp_flags.data(), grp_flags.size());
+    size_t off_thd_order = P.add(thd_order.data(), thd_order.size());
+    size_t off_thd_mod = P.add(thd_mod.data(), thd_mod.size());
+    size_t off_effns = P.add(effns.data(), effns.size());
+    size_t off_thd_pairs = P.add(thd_pairs.data(), thd_pairs.size());
+    size_t off_combo = P.add(combos.data(), combos.size());
+    size_t off_out_pos = P.add(out_pos.data(), out_pos.size());
     hipError_t e = hipMalloc(&h->d_blob, P.bytes.size());
     if (e == hipSuccess)
         e = hipMemcpy(h->d_blob, P.bytes.data(), P.bytes.size(), hipMemcpyHostToDevice);
@@ -380,16 +534,43 @@ int pyjac_hip_create(const pyjac_mech_tables* mt, int conp, char order, int jac_
     h->wt.rx_i = h->d_blob + off_rx_i;
     h->wt.rx_d = h->d_blob + off_rx_d;
     h->wt.rx_s = h->d_blob + off_rx_s;
+    h->bt.rec_i = h->d_blob + off_rec_i;
+    h->bt.rec_d = h->d_blob + off_rec_d;
+    h->bt.rec_s = h->d_blob + off_rec_s;
+    h->bt.grp_flags = (const int*)(h->d_blob + off_grp_flags);
+    h->bt.thd_order = (const int*)(h->d_blob + off_thd_order);
+    h->bt.thd_mod = (const double*)(h->d_blob + off_thd_mod);
+    h->bt.effns = (const double*)(h->d_blob + off_effns);
+    h->bt.thd_pairs = (const int*)(h->d_blob + off_thd_pairs);
+    h->bt.combo = h->d_blob + off_combo;
+    h->bt.n_pairs = (int)pairs.size();
+    h->bt.n_combo = (int)combos.size();
+    h->bt.out_pos = h->d_blob + off_out_pos;
     // the wave-per-state kernels keep a state's arrays in 64 KiB of LDS at most
     const size_t lds_cap = 64 * 1024;
     h->wave_ok_rates = packable && pjk_host::wave_lds_bytes(m, false, 0) <= lds_cap;
     h->wave_ok_jac = packable && pjk_host::wave_lds_bytes(m, true, NN * NN) <= lds_cap;
+    // the block kernels fit one workgroup's arrays into the CU's LDS
+    h->block_ok_rates = packable && pjk_host::block_lds_bytes(m, false, h->bt.n_pairs) <= LDS_PER_CU;
+    h->block_ok_jac = packable && pjk_host::block_lds_bytes(m, true, h->bt.n_pairs) <= LDS_PER_CU;
+    for (int k = 0; k < 2; ++k) {
+        h->lds[k] = pjk_host::block_lds(m, k == 1, h->bt.n_pairs);
+        pjk_host::BlockArgs a{m, h->bt, h->lds[k]};
+        hipError_t e2 = hipMalloc(&h->d_args[k], sizeof(a));
+        if (e2 == hipSuccess) e2 = hipMemcpy(h->d_args[k], &a, sizeof(a), hipMemcpyHostToDevice);
+        if (e2 != hipSuccess) {
+            pyjac_hip_destroy(h);
+            return fail(PYJAC_ERR_HIP, "argument upload failed: %s", hipGetErrorString(e2));
+        }
+    }
     *out = h;
     return PYJAC_OK;
 }
 
 int pyjac_hip_destroy(pyjac_handle* h) {
     if (!h) return PYJAC_OK;
+    (void)hipFree(h->d_args[0]);
+    (void)hipFree(h->d_args[1]);
     (void)hipFree(h->d_blob);
     (void)hipFree(h->d_ws);
     (void)hipFree(h->d_param);

@@ -1,7 +1,8 @@
-"""Both kernel families agree with the checker: the wave-per-state kernels
-(selected automatically when a state's arrays fit in LDS) and the
-lane-per-state kernels (the fallback for large mechanisms, forced here with
-variant 1)."""
+"""Every kernel family agrees with the checker: the block-per-state kernels
+with 1, 2, 4 and 8 waves per state (variants 11, 12, 14, 18; the automatic
+choice when a state's arrays fit in LDS), the wave-per-state kernels
+(variant 2) and the lane-per-state kernels (variant 1, the fallback for
+large mechanisms)."""
 
 import numpy as np
 import pytest
@@ -13,7 +14,7 @@ from pyjac.pywrap import PyJacobianKernel, PySpeciesRatesKernel
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize('variant', [1, 2])
+@pytest.mark.parametrize('variant', [1, 2, 11, 12, 14, 18])
 @pytest.mark.parametrize('name,conp,order,fmt', [('gri30', False, 'F', 1), ('test', True, 'C', 0),
                                                  ('h2o2', True, 'F', 0)])
 def test_variant_parity(mechs, variant, name, conp, order, fmt):

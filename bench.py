@@ -37,7 +37,26 @@ def parse():
     ap.add_argument('--cpu-seconds', type=float, default=15.0,
                     help='approximate CPU work of the CPU-baseline sample')
     ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--copy-bytes', type=int, default=0,
+                    help='run one device-to-device copy of this many bytes first (a kernel of '
+                         'known traffic, to calibrate hardware byte counters against)')
     return ap.parse_args()
+
+
+TRAFFIC_FILE = os.path.join(ROOT, 'profiles', 'traffic_jacobian_gri30_conv_f_sparse.txt')
+
+
+def measured_traffic(n):
+    """HBM bytes per Jacobian launch from the committed counter summary
+    (FETCH_SIZE + WRITE_SIZE per state, scaled by the copy-kernel factor),
+    or None when no summary is there."""
+    try:
+        for line in open(TRAFFIC_FILE):
+            if line.startswith('traffic_bytes_per_state'):
+                return float(line.split('=')[1]) * n
+    except OSError:
+        pass
+    return None
 
 
 def cpu_baseline(mech, budget_s):
@@ -81,6 +100,13 @@ def main():
     from pyjac.pywrap import PyJacobianKernel, PySpeciesRatesKernel
 
     dev = torch.device('cuda:0')
+    if args.copy_bytes:
+        src = torch.empty(args.copy_bytes // 8, dtype=torch.float64, device=dev).fill_(1.0)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     mech = MechTables.from_file(*mechanism_files('gri30'))
     n = args.states
     param, phi = sample_states(mech, n, seed=20240, T_range=(1000.0, 2800.0),
@@ -140,8 +166,9 @@ def main():
         'jacobian_ms': jac_ms,
         'species_rates_ms': rates_ms,
         'roofline': {'bound': 'hbm', 'achieved': achieved, 'peak': HBM_PEAK_GBS,
-                     'unit': 'GB/s', 'frac': achieved / HBM_PEAK_GBS, 'traffic': None,
-                     'kernel': 'jacobian_kernel',
+                     'unit': 'GB/s', 'frac': achieved / HBM_PEAK_GBS,
+                     'traffic': measured_traffic(n),
+                     'kernel': 'jacobian_block_kernel',
                      'algorithmic_bytes_per_state': bytes_per_state},
     }
     if not args.no_cpu:

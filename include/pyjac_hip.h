@@ -191,8 +191,11 @@ int pyjac_hip_jacobian_size(const pyjac_handle *h); /* NN*NN or nnz */
    (Kernel::requiredMemorySize, kernel.hpp.in:220). */
 size_t pyjac_hip_required_memory(const pyjac_handle *h);
 
-/* Select the kernel variant: 0 automatic (tuned), else a variant id
-   returned by pyjac_hip_variant_count / used by the autotuner. */
+/* Kernel variant of one handle; no counterpart in the reference, whose
+   generated code fixes the layout at generation time (--width/--depth).
+   kernel: 0 species rates, 1 Jacobian, 2 chem utils.  variant: 0 automatic
+   (the tuned choice), 1 lane-per-state kernels, 2 wave-per-state kernels,
+   10 + W block-per-state kernels with W = 1, 2, 4 or 8 waves per state. */
 int pyjac_hip_set_variant(pyjac_handle *h, int kernel, int variant);
 int pyjac_hip_get_variant(const pyjac_handle *h, int kernel);
 

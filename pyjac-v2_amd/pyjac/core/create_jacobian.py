@@ -1,0 +1,97 @@
+"""The generator's entry point for the HIP target.
+
+The reference's ``create_jacobian`` (``core/create_jacobian.py:5212-5420``)
+parses the mechanism, builds the tables and emits C/OpenCL source plus a
+Python wrapper module.  The HIP backend needs no generated code: the one
+library takes any mechanism as tables at run time.  So for ``lang='hip'``
+this writes into ``build_path``
+
+* ``pyjac_hip.py`` -- the wrapper module, exposing ``PyJacobianKernel``,
+  ``PySpeciesRatesKernel`` and ``PyChemUtilsKernel`` with the reference's
+  constructor ``(problem_size, work_size, do_not_compile=False)``, bound to
+  this mechanism and these options;
+* the mechanism (and thermo) file it was made from, and ``pyjac_hip.json``
+  with the options;
+* ``libpyjac_hip.so``, copied from the package build (built on demand).
+"""
+
+import json
+import logging
+import os
+import shutil
+
+from .enum_types import JacobianFormat, JacobianType, KernelType, RateSpecialization
+from .exceptions import InvalidInputSpecificationException
+from .mech_tables import MechTables
+
+MODULE = '''"""pyJac kernels for {mech} (HIP target), generated by pyjac.
+
+Options: conp={conp}, order={order}, jac_format={jac_format}, jac_type={jac_type},
+rate_specialization={rate_spec}, kernel_type={kernel_type}.
+"""
+import json as _json
+import os as _os
+
+_here = _os.path.dirname(_os.path.abspath(__file__))
+with open(_os.path.join(_here, 'pyjac_hip.json')) as _f:
+    _opts = _json.load(_f)
+_os.environ.setdefault('PYJAC_HIP_LIB', _os.path.join(_here, 'libpyjac_hip.so'))
+
+from pyjac.core.mech_tables import MechTables as _MechTables  # noqa: E402
+from pyjac.pywrap import bind as _bind  # noqa: E402
+
+mech = _MechTables.from_file(
+    _os.path.join(_here, _opts['mech']),
+    _os.path.join(_here, _opts['thermo']) if _opts['thermo'] else None,
+    last_spec=_opts['last_spec'], rate_spec=_opts['rate_spec'], jac_type=_opts['jac_type'])
+globals().update(_bind(mech, conp=_opts['conp'], order=_opts['order'],
+                       jac_format=_opts['jac_format'], jac_type=_opts['jac_type']))
+species_names = list(mech.species_names)
+reaction_strings = list(mech.reaction_strings)
+'''
+
+
+def create_jacobian(lang, mech_name=None, therm_name=None, build_path='./out/',
+                    last_spec=None, kernel_type=KernelType.jacobian, data_order='C',
+                    rate_specialization=RateSpecialization.hybrid, conp=True,
+                    jac_type=JacobianType.exact, jac_format=JacobianFormat.sparse,
+                    build_library=True, **ignored):
+    """Write the HIP kernel module for one mechanism; returns its path."""
+    if lang != 'hip':
+        raise NotImplementedError('this build provides the hip target only')
+    if mech_name is None:
+        raise InvalidInputSpecificationException('mech_name')
+    if data_order not in ('C', 'F'):
+        raise InvalidInputSpecificationException('data_order')
+    jac_type = JacobianType(jac_type)
+    if jac_type != JacobianType.exact:
+        raise NotImplementedError('the HIP target computes exact Jacobians only')
+    # parse and build the tables now, so that a bad mechanism fails here
+    mech = MechTables.from_file(mech_name, therm_name, last_spec=last_spec,
+                                rate_spec=rate_specialization, jac_type=jac_type)
+    os.makedirs(build_path, exist_ok=True)
+    mech_base = os.path.basename(mech_name)
+    shutil.copy2(mech_name, os.path.join(build_path, mech_base))
+    therm_base = None
+    if therm_name:
+        therm_base = os.path.basename(therm_name)
+        shutil.copy2(therm_name, os.path.join(build_path, therm_base))
+    opts = dict(mech=mech_base, thermo=therm_base, last_spec=last_spec,
+                rate_spec=int(RateSpecialization(rate_specialization)),
+                jac_type=int(jac_type), jac_format=int(JacobianFormat(jac_format)),
+                conp=bool(conp), order=data_order, kernel_type=KernelType(kernel_type).name)
+    with open(os.path.join(build_path, 'pyjac_hip.json'), 'w') as f:
+        json.dump(opts, f, indent=1)
+    path = os.path.join(build_path, 'pyjac_hip.py')
+    with open(path, 'w') as f:
+        f.write(MODULE.format(mech=mech.name, conp=opts['conp'], order=data_order,
+                              jac_format=opts['jac_format'], jac_type=opts['jac_type'],
+                              rate_spec=opts['rate_spec'], kernel_type=opts['kernel_type']))
+    if build_library:
+        from ..libgen import LIB_PATH, build_library as _build
+        if not os.path.exists(LIB_PATH):
+            _build()
+        shutil.copy2(LIB_PATH, os.path.join(build_path, os.path.basename(LIB_PATH)))
+    logging.getLogger(__name__).info('wrote %s (%d species, %d reactions)', path, mech.Ns,
+                                     mech.Nr)
+    return path

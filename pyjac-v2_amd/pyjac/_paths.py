@@ -13,6 +13,11 @@ MECHANISMS = {
 
 
 def mechanism_files(name):
+    """(mechanism, thermo) paths of a bundled mechanism, or of a seeded
+    synthetic one ('s150', 's500'), generated on first use."""
+    if name in ('s150', 's500'):
+        from .core.synthetic import workload_file
+        return workload_file(name), None
     mech, therm = MECHANISMS[name]
     return (os.path.join(MECH_DIR, mech),
             os.path.join(MECH_DIR, therm) if therm else None)

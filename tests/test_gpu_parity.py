@@ -80,3 +80,32 @@ def test_chem_utils(mechs, name, conp):
     k(phi, b, e, h)
     for got, ref in ((e, e_ref), (h, h_ref), (b, b_ref)):
         assert np.allclose(got, ref, rtol=1e-12, atol=1e-12 * np.abs(ref).max())
+
+
+LARGE = [('s150', True, 'C', 0), ('s150', False, 'F', 1), ('s500', True, 'F', 1),
+         ('s500', False, 'C', 1)]
+
+
+@pytest.mark.parametrize('name,conp,order,fmt', LARGE)
+def test_large_mechanisms(mechs, name, conp, order, fmt):
+    """Seeded synthetic mechanisms of 150 and 500 species: species rates and
+    Jacobians (whose dense accumulator exceeds LDS, so the lane-per-state
+    kernels take them)."""
+    mech = mechs(name)
+    n = 96
+    param, phi, phi_in = _states(mech, n, conp, order, seed=17)
+    ref = oracle.species_rates(mech, conp, param, phi)
+    rk = PySpeciesRatesKernel(n, 0, mech=mech, conp=conp, order=order)
+    out = np.zeros_like(phi_in)
+    rk(param, phi_in, out)
+    got = out if order == 'C' else out.T
+    assert _rate_err(got, ref) <= 1.0
+    dense = oracle.jacobian(mech, conp, param, phi)
+    jref = oracle.to_layout(mech, dense, order, fmt)
+    jk = PyJacobianKernel(n, 0, mech=mech, conp=conp, order=order, jac_format=fmt)
+    jac = np.zeros_like(jref)
+    jk(param, phi_in, jac)
+    a = jac.reshape(n, -1) if order == 'C' else jac.reshape(-1, n).T
+    b = jref.reshape(n, -1) if order == 'C' else jref.reshape(-1, n).T
+    fro = np.linalg.norm(a - b, axis=1) / np.linalg.norm(b, axis=1)
+    assert fro.max() <= JAC_FRO, fro.max()

@@ -174,7 +174,8 @@ def textbook_rates(mech, conp, param, phi):
     return dphi, dphi_scale, qnet, wdot, scale
 
 
-CASES = [(m, conp) for m in ('h2o2', 'gri30', 'test') for conp in (True, False)]
+CASES = [(m, conp) for m in ('h2o2', 'gri30', 'test') for conp in (True, False)] + \
+    [('s150', True), ('s500', False)]
 
 
 @pytest.mark.parametrize('name,conp', CASES)

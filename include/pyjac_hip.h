@@ -199,6 +199,34 @@ size_t pyjac_hip_required_memory(const pyjac_handle *h);
 int pyjac_hip_set_variant(pyjac_handle *h, int kernel, int variant);
 int pyjac_hip_get_variant(const pyjac_handle *h, int kernel);
 
+/*
+ * Ragged batches (no counterpart in the reference, which builds one kernel
+ * per mechanism): several handles -- different mechanisms or option sets --
+ * each with its own resident states, evaluated by one call.  Members that the
+ * block-per-state kernels take share one launch (up to 8 per launch); the
+ * others get their own launches.  With use_graph the launches are captured
+ * into a HIP graph on the first call after a change and replayed afterwards.
+ * kernel: 0 species rates, 1 Jacobian.  Calls on one batch are not
+ * re-entrant.
+ */
+typedef struct pyjac_batch pyjac_batch;
+int pyjac_hip_batch_create(pyjac_handle *const *handles, int count, pyjac_batch **out);
+int pyjac_hip_batch_set(pyjac_batch *b, int member, size_t n_states, const double *d_param,
+                        const double *d_phi, double *d_out);
+int pyjac_hip_batch_run(pyjac_batch *b, int kernel, int use_graph, void *stream);
+int pyjac_hip_batch_destroy(pyjac_batch *b);
+
+/*
+ * Autotuner (no counterpart in the reference): times every kernel family that
+ * can run the handle's mechanism -- block-per-state with 1, 2, 4 or 8 waves
+ * per state, wave-per-state, lane-per-state -- on n resident states (results
+ * land in d_out) and makes the fastest the handle's automatic choice.
+ * Reports the winning variant id and its time per call.
+ */
+int pyjac_hip_autotune(pyjac_handle *h, int kernel, size_t n, const double *d_param,
+                       const double *d_phi, double *d_out, void *stream, int *best_variant,
+                       float *best_ms);
+
 #ifdef __cplusplus
 }
 #endif

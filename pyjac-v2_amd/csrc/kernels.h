@@ -85,4 +85,19 @@ hipError_t launch_jacobian_block(const DevMech& m, const BlockArgs* args, const 
                                  int wps, int conp, char order, long n, const double* param,
                                  const double* phi, double* jac, hipStream_t stream);
 
+// ---- ragged batch: several handles' states in one block-kernel launch
+constexpr int RAGGED_MAX = 8;
+struct RaggedLaunch {
+    const BlockArgs* args;  // device copy of the member's BlockArgs (its kernel's)
+    int conp, NN;
+    char order;
+    long n;
+    const double *param, *phi;
+    double* out;
+};
+
+// kernel 0: species rates, 1: Jacobian; lds = the largest member's bytes
+hipError_t launch_ragged(int kernel, int wps, int count, const RaggedLaunch* members,
+                         size_t lds, hipStream_t stream);
+
 }  // namespace pjk_host

@@ -835,17 +835,15 @@ __device__ __forceinline__ State load_state(int conp, double param, double x0, d
     return st;
 }
 
+// species rates of state s (one workgroup of 64 * WPS threads)
 template <int WPS>
-__global__ void __launch_bounds__(64 * WPS)
-rates_block_kernel(const BlockArgs* __restrict__ A, int conp, long n,
-                   const double* __restrict__ param, const double* __restrict__ phi,
-                   double* __restrict__ dphi, Layout L) {
+__device__ __forceinline__ void rates_state(const BlockArgs* __restrict__ A, int conp, long n,
+                                            long s, const double* __restrict__ param,
+                                            const double* __restrict__ phi,
+                                            double* __restrict__ dphi, Layout L, double* smem) {
     constexpr int NT = 64 * WPS;
-    extern __shared__ double smem[];
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const long s = state_of_block(blockIdx.x, n);
-    if (s >= n) return;
     Ctx c = make_ctx(A, smem, conp);
     const int Ns = c.Ns, Nr = c.m.Nr;
     if (tid == 0) { c.conc(Ns) = 1.0; c.b(Ns) = 0.0; }
@@ -902,17 +900,16 @@ rates_block_kernel(const BlockArgs* __restrict__ A, int conp, long n,
     }
 }
 
+// analytical Jacobian of state s (one workgroup of 64 * WPS threads)
 template <int WPS>
-__global__ void __launch_bounds__(64 * WPS)
-jacobian_block_kernel(const BlockArgs* __restrict__ A, int conp, long n,
-                      const double* __restrict__ param, const double* __restrict__ phi,
-                      double* __restrict__ jac, Layout L, int out_C) {
+__device__ __forceinline__ void jacobian_state(const BlockArgs* __restrict__ A, int conp, long n,
+                                               long s, const double* __restrict__ param,
+                                               const double* __restrict__ phi,
+                                               double* __restrict__ jac, Layout L, int out_C,
+                                               double* smem) {
     constexpr int NT = 64 * WPS;
-    extern __shared__ double smem[];
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const long s = state_of_block(blockIdx.x, n);
-    if (s >= n) return;
     Ctx c = make_ctx(A, smem, conp);
     const int Ns = c.Ns, NN = c.NN, ns = c.ns, Nr = c.m.Nr, nout = c.m.jac_nnz;
     lds_double* J = c.J();
@@ -1091,6 +1088,71 @@ jacobian_block_kernel(const BlockArgs* __restrict__ A, int conp, long n,
     }
 }
 
+template <int WPS>
+__global__ void __launch_bounds__(64 * WPS)
+rates_block_kernel(const BlockArgs* __restrict__ A, int conp, long n,
+                   const double* __restrict__ param, const double* __restrict__ phi,
+                   double* __restrict__ dphi, Layout L) {
+    extern __shared__ double smem[];
+    const long s = state_of_block(blockIdx.x, n);
+    if (s >= n) return;
+    rates_state<WPS>(A, conp, n, s, param, phi, dphi, L, smem);
+}
+
+template <int WPS>
+__global__ void __launch_bounds__(64 * WPS)
+jacobian_block_kernel(const BlockArgs* __restrict__ A, int conp, long n,
+                      const double* __restrict__ param, const double* __restrict__ phi,
+                      double* __restrict__ jac, Layout L, int out_C) {
+    extern __shared__ double smem[];
+    const long s = state_of_block(blockIdx.x, n);
+    if (s >= n) return;
+    jacobian_state<WPS>(A, conp, n, s, param, phi, jac, L, out_C, smem);
+}
+
+// Ragged batch: states of several mechanisms / option sets in one launch.
+// Member i owns workgroups [block0_i, block0_i + 8 ceil(n_i / 8)); inside
+// its range the XCD-aware state order of the single-mechanism launch applies.
+struct RaggedMember {
+    const BlockArgs* A;
+    int conp, out_C;
+    long n, block0;
+    const double *param, *phi;
+    double* out;
+    Layout L;
+};
+struct RaggedArgs {
+    int count;
+    RaggedMember m[pjk_host::RAGGED_MAX];
+};
+
+__device__ __forceinline__ int ragged_member(const RaggedArgs& R, long b) {
+    int i = 0;
+    for (int k = 1; k < R.count; ++k)
+        if (R.m[k].block0 <= b) i = k;
+    return i;
+}
+
+template <int WPS>
+__global__ void __launch_bounds__(64 * WPS) rates_ragged_kernel(RaggedArgs R) {
+    extern __shared__ double smem[];
+    const int i = ragged_member(R, blockIdx.x);
+    const RaggedMember& M = R.m[i];
+    const long s = state_of_block(blockIdx.x - M.block0, M.n);
+    if (s >= M.n) return;
+    rates_state<WPS>(M.A, M.conp, M.n, s, M.param, M.phi, M.out, M.L, smem);
+}
+
+template <int WPS>
+__global__ void __launch_bounds__(64 * WPS) jacobian_ragged_kernel(RaggedArgs R) {
+    extern __shared__ double smem[];
+    const int i = ragged_member(R, blockIdx.x);
+    const RaggedMember& M = R.m[i];
+    const long s = state_of_block(blockIdx.x - M.block0, M.n);
+    if (s >= M.n) return;
+    jacobian_state<WPS>(M.A, M.conp, M.n, s, M.param, M.phi, M.out, M.L, M.out_C, smem);
+}
+
 }  // namespace pjb
 
 namespace pjk_host {
@@ -1190,6 +1252,47 @@ hipError_t launch_jacobian_block(const DevMech& m, const BlockArgs* args, const 
         default: return hipErrorInvalidValue;
     }
 #undef PJB_JAC
+    return hipGetLastError();
+}
+
+hipError_t launch_ragged(int kernel, int wps, int count, const RaggedLaunch* members,
+                         size_t lds, hipStream_t stream) {
+    if (count < 1 || count > RAGGED_MAX) return hipErrorInvalidValue;
+    pjb::RaggedArgs R{};
+    R.count = count;
+    long blocks = 0;
+    for (int i = 0; i < count; ++i) {
+        const RaggedLaunch& q = members[i];
+        pjb::Layout L = q.order == 'F' ? pjb::Layout{1, (size_t)q.n}
+                                       : pjb::Layout{(size_t)q.NN, 1};
+        R.m[i] = pjb::RaggedMember{q.args, q.conp, q.order == 'C', q.n, blocks, q.param, q.phi,
+                                   q.out, L};
+        blocks += 8 * ((q.n + 7) / 8);
+    }
+    if (blocks == 0) return hipSuccess;
+#define PJB_RAG(W)                                                                        \
+    case W: {                                                                             \
+        if (kernel == 0) {                                                                \
+            hipError_t e = set_lds(pjb::rates_ragged_kernel<W>, lds);                    \
+            if (e != hipSuccess) return e;                                                \
+            hipLaunchKernelGGL(pjb::rates_ragged_kernel<W>, dim3(blocks), dim3(64 * W),   \
+                               lds, stream, R);                                           \
+        } else {                                                                          \
+            hipError_t e = set_lds(pjb::jacobian_ragged_kernel<W>, lds);                 \
+            if (e != hipSuccess) return e;                                                \
+            hipLaunchKernelGGL(pjb::jacobian_ragged_kernel<W>, dim3(blocks), dim3(64 * W), \
+                               lds, stream, R);                                           \
+        }                                                                                 \
+        break;                                                                            \
+    }
+    switch (wps) {
+        PJB_RAG(1)
+        PJB_RAG(2)
+        PJB_RAG(4)
+        PJB_RAG(8)
+        default: return hipErrorInvalidValue;
+    }
+#undef PJB_RAG
     return hipGetLastError();
 }
 

@@ -150,6 +150,14 @@ bool use_block(const pyjac_handle* h, int kernel) {
     return ok && (v == 0 || v > 10);
 }
 
+// workspace of the lane-per-state kernels for n states, allocated ahead of
+// a graph capture (no allocation may happen while a stream is captured)
+int prepare_lane(pyjac_handle* h, int kernel, long n) {
+    size_t per = pjk_host::ws_doubles_per_state(h->m, kernel == 1);
+    int grid = pick_grid(h, per, h->variant[kernel], n);
+    return ensure_ws(h, per * (size_t)grid * h->block * sizeof(double));
+}
+
 int run_rates(pyjac_handle* h, long n, const double* param, const double* phi, double* dphi,
               hipStream_t s) {
     if (n <= 0) return PYJAC_OK;
@@ -661,6 +669,200 @@ int pyjac_hip_jacobian_size(const pyjac_handle* h) { return h ? nvalues(h) : -1;
 size_t pyjac_hip_required_memory(const pyjac_handle* h) {
     if (!h) return 0;
     return pjk_host::ws_doubles_per_state(h->m, true) * sizeof(double);
+}
+
+// ------------------------------------------------------------ ragged batch
+}  // extern "C"
+
+struct pyjac_batch {
+    std::vector<pyjac_handle*> h;
+    struct Member {
+        size_t n = 0;
+        const double *param = nullptr, *phi = nullptr;
+        double* out = nullptr;
+    };
+    std::vector<Member> mem;
+    hipStream_t cap = nullptr;            // capture stream
+    hipGraph_t graph[2] = {nullptr, nullptr};
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
+    bool dirty[2] = {true, true};
+};
+
+namespace {
+
+void drop_graph(pyjac_batch* b, int k) {
+    if (b->exec[k]) (void)hipGraphExecDestroy(b->exec[k]);
+    if (b->graph[k]) (void)hipGraphDestroy(b->graph[k]);
+    b->exec[k] = nullptr;
+    b->graph[k] = nullptr;
+    b->dirty[k] = true;
+}
+
+// every launch of one batch evaluation, on stream s
+int batch_launches(pyjac_batch* b, int kernel, hipStream_t s) {
+    std::vector<pjk_host::RaggedLaunch> rag;
+    size_t lds = 0;
+    int wps = 1;
+    for (size_t i = 0; i < b->h.size(); ++i) {
+        pyjac_handle* h = b->h[i];
+        const auto& m = b->mem[i];
+        if (!m.n) continue;
+        if (use_block(h, kernel) && rag.size() < (size_t)pjk_host::RAGGED_MAX) {
+            rag.push_back(pjk_host::RaggedLaunch{h->d_args[kernel], h->conp, h->m.NN, h->order,
+                                                 (long)m.n, m.param, m.phi, m.out});
+            lds = std::max(lds, (size_t)h->lds[kernel].total * sizeof(double));
+            wps = std::max(wps, block_wps(h, kernel));
+        } else {
+            int rc = kernel == 0 ? run_rates(h, (long)m.n, m.param, m.phi, m.out, s)
+                                 : run_jac(h, (long)m.n, m.param, m.phi, m.out, s);
+            if (rc) return rc;
+        }
+    }
+    if (!rag.empty())
+        HIP_TRY(pjk_host::launch_ragged(kernel, wps, (int)rag.size(), rag.data(), lds, s));
+    return PYJAC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pyjac_hip_batch_create(pyjac_handle* const* handles, int count, pyjac_batch** out) {
+    if (!handles || !out || count < 1) return fail(PYJAC_ERR_ARG, "null argument or empty batch");
+    *out = nullptr;
+    for (int i = 0; i < count; ++i)
+        if (!handles[i]) return fail(PYJAC_ERR_ARG, "null handle %d", i);
+    pyjac_batch* b = new pyjac_batch();
+    b->h.assign(handles, handles + count);
+    b->mem.resize(count);
+    if (hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking) != hipSuccess) {
+        delete b;
+        return fail(PYJAC_ERR_HIP, "stream creation failed");
+    }
+    *out = b;
+    return PYJAC_OK;
+}
+
+int pyjac_hip_batch_set(pyjac_batch* b, int i, size_t n_states, const double* d_param,
+                        const double* d_phi, double* d_out) {
+    if (!b || i < 0 || i >= (int)b->h.size()) return fail(PYJAC_ERR_ARG, "bad batch member");
+    if (n_states && (!d_param || !d_phi || !d_out)) return fail(PYJAC_ERR_ARG, "null argument");
+    auto& m = b->mem[i];
+    if (m.n != n_states || m.param != d_param || m.phi != d_phi || m.out != d_out) {
+        m.n = n_states;
+        m.param = d_param;
+        m.phi = d_phi;
+        m.out = d_out;
+        drop_graph(b, 0);
+        drop_graph(b, 1);
+    }
+    return PYJAC_OK;
+}
+
+int pyjac_hip_batch_run(pyjac_batch* b, int kernel, int use_graph, void* stream) {
+    if (!b || (kernel != 0 && kernel != 1)) return fail(PYJAC_ERR_ARG, "bad batch call");
+    hipStream_t s = (hipStream_t)stream;
+    if (!use_graph) return batch_launches(b, kernel, s);
+    if (b->dirty[kernel]) {
+        // allocations and kernel attributes first: nothing may allocate
+        // while the stream is captured
+        for (size_t i = 0; i < b->h.size(); ++i) {
+            pyjac_handle* h = b->h[i];
+            const bool wave = h->variant[kernel] == 2 &&
+                              (kernel == 0 ? h->wave_ok_rates : h->wave_ok_jac);
+            if (b->mem[i].n && !use_block(h, kernel) && !wave) {
+                int rc = prepare_lane(h, kernel, (long)b->mem[i].n);
+                if (rc) return rc;
+            }
+        }
+        int rc = batch_launches(b, kernel, b->cap);  // eager run: sets kernel attributes
+        if (rc) return rc;
+        HIP_TRY(hipStreamSynchronize(b->cap));
+        HIP_TRY(hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal));
+        rc = batch_launches(b, kernel, b->cap);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(b->cap, &g);
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (e != hipSuccess) return fail(PYJAC_ERR_HIP, "capture failed: %s", hipGetErrorString(e));
+        b->graph[kernel] = g;
+        HIP_TRY(hipGraphInstantiate(&b->exec[kernel], g, nullptr, nullptr, 0));
+        b->dirty[kernel] = false;
+    }
+    HIP_TRY(hipGraphLaunch(b->exec[kernel], s));
+    return PYJAC_OK;
+}
+
+int pyjac_hip_batch_destroy(pyjac_batch* b) {
+    if (!b) return PYJAC_OK;
+    drop_graph(b, 0);
+    drop_graph(b, 1);
+    if (b->cap) (void)hipStreamDestroy(b->cap);
+    delete b;
+    return PYJAC_OK;
+}
+
+// ---------------------------------------------------------------- autotuner
+// Times the kernel families that can run this handle's mechanism on n
+// resident states and keeps the fastest: block-per-state with 1, 2, 4 or 8
+// waves per state, wave-per-state, lane-per-state.
+int pyjac_hip_autotune(pyjac_handle* h, int kernel, size_t n, const double* d_param,
+                       const double* d_phi, double* d_out, void* stream, int* best_variant,
+                       float* best_ms) {
+    if (!h || (kernel != 0 && kernel != 1) || !n || !d_param || !d_phi || !d_out)
+        return fail(PYJAC_ERR_ARG, "bad autotune call");
+    hipStream_t s = (hipStream_t)stream;
+    std::vector<int> cand;
+    bool block = kernel == 0 ? h->block_ok_rates : h->block_ok_jac;
+    bool wave = kernel == 0 ? h->wave_ok_rates : h->wave_ok_jac;
+    if (block)
+        for (int w : {1, 2, 4, 8}) cand.push_back(10 + w);
+    if (wave) cand.push_back(2);
+    cand.push_back(1);
+    const int saved_variant = h->variant[kernel], saved_wps = h->wps[kernel];
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    int best = -1;
+    float best_t = 0.0f;
+    int rc = PYJAC_OK;
+    for (int v : cand) {
+        h->variant[kernel] = v;
+        auto run = [&]() {
+            return kernel == 0 ? run_rates(h, (long)n, d_param, d_phi, d_out, s)
+                               : run_jac(h, (long)n, d_param, d_phi, d_out, s);
+        };
+        if ((rc = run())) break;
+        if (hipEventRecord(e0, s) != hipSuccess) { rc = fail(PYJAC_ERR_HIP, "event"); break; }
+        for (int r = 0; r < 3 && !rc; ++r) rc = run();
+        if (rc) break;
+        if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
+            rc = fail(PYJAC_ERR_HIP, "event");
+            break;
+        }
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        ms /= 3.0f;
+        if (best < 0 || ms < best_t) { best = v; best_t = ms; }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) {
+        h->variant[kernel] = saved_variant;
+        h->wps[kernel] = saved_wps;
+        return rc;
+    }
+    if (best > 10) {
+        h->variant[kernel] = 0;
+        h->wps[kernel] = best - 10;
+    } else {
+        h->variant[kernel] = best;
+    }
+    if (best_variant) *best_variant = best;
+    if (best_ms) *best_ms = best_t;
+    return PYJAC_OK;
 }
 
 int pyjac_hip_set_variant(pyjac_handle* h, int kernel, int variant) {

@@ -59,6 +59,12 @@ def load_library(path=None):
     L.pyjac_hip_required_memory.restype = S
     L.pyjac_hip_set_variant.argtypes = [P, I, I]
     L.pyjac_hip_get_variant.argtypes = [P, I]
+    L.pyjac_hip_batch_create.argtypes = [ctypes.POINTER(P), I, ctypes.POINTER(P)]
+    L.pyjac_hip_batch_set.argtypes = [P, I, S, P, P, P]
+    L.pyjac_hip_batch_run.argtypes = [P, I, I, P]
+    L.pyjac_hip_batch_destroy.argtypes = [P]
+    L.pyjac_hip_autotune.argtypes = [P, I, S, P, P, P, P, ctypes.POINTER(I),
+                                     ctypes.POINTER(ctypes.c_float)]
     if path == LIB_PATH or _lib is None:
         _lib = L
     return L
@@ -116,6 +122,8 @@ class _KernelBase(object):
                                         ctypes.byref(h)))
         self._h = h
         self.problem_size = 0
+        if self._variant_slot <= 1:
+            self._apply_tuned()
         if not kwargs.pop('testing_only_skip_compilation', False):
             self.resize(problem_size, work_size, do_not_compile)
 
@@ -151,6 +159,46 @@ class _KernelBase(object):
     def set_variant(self, variant):
         check(self.lib.pyjac_hip_set_variant(self._h, self._variant_slot, int(variant)))
 
+    def autotune(self, param, phi, out, cache=True):
+        """Time the kernel families on these resident states (device
+        arrays; out receives a result) and keep the fastest.  The choice is
+        remembered in a JSON cache keyed by mechanism, options and device
+        (``PYJAC_HIP_TUNE_CACHE``, default ~/.cache/pyjac_hip/autotune.json)
+        and reapplied by later kernels of the same key.  Returns
+        (variant, milliseconds per call)."""
+        if self._variant_slot > 1:
+            raise InvalidInputSpecificationException('autotune: rates or Jacobian only')
+        pp, dp = _ptr(param)
+        ph, dph = _ptr(phi)
+        po, do = _ptr(out)
+        if not (dp and dph and do):
+            raise ValueError('autotune needs device arrays')
+        v = ctypes.c_int()
+        ms = ctypes.c_float()
+        check(self.lib.pyjac_hip_autotune(self._h, self._variant_slot, _numel(param), pp, ph, po,
+                                          _current_stream_ptr(), ctypes.byref(v), ctypes.byref(ms)))
+        if cache:
+            _tune_store(self._tune_key(), int(v.value), float(ms.value))
+        return int(v.value), float(ms.value)
+
+    def _tune_key(self):
+        import hashlib
+        try:
+            import torch
+            dev = torch.cuda.get_device_name(0)
+        except Exception:
+            dev = 'unknown'
+        h = hashlib.sha1('\n'.join(self.mech.species_names + self.mech.reaction_strings)
+                         .encode()).hexdigest()[:16]
+        return '{}|{}|conp={}|{}|fmt={}|{}'.format(h, self.kernel_type, int(self.conp),
+                                                   self.order, self.jac_format, dev)
+
+    def _apply_tuned(self):
+        choice = _tune_load().get(self._tune_key())
+        if choice is not None:
+            check(self.lib.pyjac_hip_set_variant(self._h, self._variant_slot,
+                                                 int(choice['variant'])))
+
     def _run(self, host_fn, dev_fn, param, phi, out):
         pp, dp = _ptr(param)
         ph, dph = _ptr(phi)
@@ -167,6 +215,72 @@ class _KernelBase(object):
             self.resize(n)
         check(host_fn(self._h, pp, ph, po))
         return out
+
+
+def _tune_path():
+    return os.environ.get('PYJAC_HIP_TUNE_CACHE',
+                          os.path.join(os.path.expanduser('~'), '.cache', 'pyjac_hip',
+                                       'autotune.json'))
+
+
+def _tune_load():
+    import json
+    try:
+        with open(_tune_path()) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return {}
+
+
+def _tune_store(key, variant, ms):
+    import json
+    d = _tune_load()
+    d[key] = {'variant': variant, 'ms': ms}
+    path = _tune_path()
+    os.makedirs(os.path.dirname(path), exist_ok=True)
+    tmp = path + '.%d.tmp' % os.getpid()
+    with open(tmp, 'w') as f:
+        json.dump(d, f, indent=1)
+    os.replace(tmp, path)
+
+
+class Batch(object):
+    """Ragged batch over several kernels (mechanisms / option sets) of the
+    same kind: ``set(i, param, phi, out)`` with device arrays per member, then
+    ``run()`` evaluates all of them -- one launch for the members the
+    block-per-state kernels take, captured with the others into one HIP graph
+    that later calls replay."""
+
+    def __init__(self, kernels):
+        if not kernels:
+            raise InvalidInputSpecificationException('kernels')
+        kinds = {k._variant_slot for k in kernels}
+        if len(kinds) != 1 or kinds.pop() > 1:
+            raise InvalidInputSpecificationException('kernels must all be rates or all Jacobian')
+        self.kernels = list(kernels)
+        self.lib = kernels[0].lib
+        arr = (ctypes.c_void_p * len(kernels))(*[k._h.value for k in kernels])
+        b = ctypes.c_void_p()
+        check(self.lib.pyjac_hip_batch_create(arr, len(kernels), ctypes.byref(b)))
+        self._b = b
+        self._keep = [None] * len(kernels)
+
+    def set(self, i, param, phi, out):
+        pp, _ = _ptr(param)
+        ph, _ = _ptr(phi)
+        po, _ = _ptr(out)
+        self._keep[i] = (param, phi, out)
+        check(self.lib.pyjac_hip_batch_set(self._b, i, _numel(param), pp, ph, po))
+
+    def run(self, graph=True):
+        check(self.lib.pyjac_hip_batch_run(self._b, self.kernels[0]._variant_slot, int(graph),
+                                           _current_stream_ptr()))
+
+    def __del__(self):
+        b = getattr(self, '_b', None)
+        if b is not None and b.value:
+            self.lib.pyjac_hip_batch_destroy(b)
+            self._b = None
 
 
 def _current_stream_ptr():

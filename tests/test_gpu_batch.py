@@ -1,0 +1,113 @@
+"""Ragged batches over several mechanisms in one launch (and one HIP graph),
+and the autotuner: results equal the single-handle calls."""
+
+import numpy as np
+import pytest
+
+import oracle
+from rate_tol import rate_error, rate_scales
+from pyjac.core.states import sample_states
+from pyjac.pywrap import Batch, PyJacobianKernel, PySpeciesRatesKernel
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip('torch')
+
+MEMBERS = [('h2o2', True, 'C', 0, 3000), ('gri30', False, 'F', 1, 2500),
+           ('test', False, 'C', 1, 1111), ('s150', True, 'F', 0, 300)]
+
+
+def _device_states(mech, n, conp, order, seed):
+    param, phi = sample_states(mech, n, seed=seed, conp=conp)
+    ph = phi if order == 'C' else np.ascontiguousarray(phi.T)
+    return torch.from_numpy(param).cuda(), torch.from_numpy(ph).cuda()
+
+
+def _setup(mechs, jac):
+    kernels, arrays = [], []
+    for i, (name, conp, order, fmt, n) in enumerate(MEMBERS):
+        mech = mechs(name)
+        K = PyJacobianKernel if jac else PySpeciesRatesKernel
+        k = K(n, 0, mech=mech, conp=conp, order=order, jac_format=fmt)
+        p, f = _device_states(mech, n, conp, order, seed=40 + i)
+        nv = (mech.nnz if fmt else mech.NN ** 2) if jac else mech.NN
+        out = torch.empty(nv * n, dtype=torch.float64, device='cuda')
+        kernels.append(k)
+        arrays.append((p, f, out))
+    return kernels, arrays
+
+
+def _eager(kernels, arrays):
+    res = []
+    for k, (p, f, _) in zip(kernels, arrays):
+        o = torch.empty_like(arrays[kernels.index(k)][2])
+        k(p, f, o)
+        res.append(o)
+    torch.cuda.synchronize()
+    return res
+
+
+@pytest.mark.parametrize('jac', [False, True])
+def test_batch_equals_single_calls(mechs, jac):
+    kernels, arrays = _setup(mechs, jac)
+    ref = _eager(kernels, arrays)
+    b = Batch(kernels)
+    for i, (p, f, o) in enumerate(arrays):
+        b.set(i, p, f, o)
+    for graph in (False, True):
+        for o in (a[2] for a in arrays):
+            o.fill_(np.nan)
+        b.run(graph=graph)
+        torch.cuda.synchronize()
+        for (_, _, o), r in zip(arrays, ref):
+            assert torch.equal(o, r)
+
+
+def test_graph_replays_follow_new_inputs(mechs):
+    """Replays of the captured graph, with eager work in between, see the
+    inputs as they are at replay time."""
+    kernels, arrays = _setup(mechs, True)
+    b = Batch(kernels)
+    for i, (p, f, o) in enumerate(arrays):
+        b.set(i, p, f, o)
+    for rep in range(3):
+        # perturb the temperatures in place (eager work), then replay
+        for (p, f, o), (name, conp, order, fmt, n) in zip(arrays, MEMBERS):
+            if order == 'C':
+                f.view(n, -1)[:, 0] *= 1.0 + 0.01 * (rep + 1)
+            else:
+                f.view(-1, n)[0, :] *= 1.0 + 0.01 * (rep + 1)
+        torch.cuda.synchronize()
+        ref = _eager(kernels, arrays)
+        b.run(graph=True)
+        torch.cuda.synchronize()
+        for (_, _, o), r in zip(arrays, ref):
+            assert torch.equal(o, r), rep
+
+
+def test_autotune_keeps_results_right(mechs, tmp_path, monkeypatch):
+    monkeypatch.setenv('PYJAC_HIP_TUNE_CACHE', str(tmp_path / 'tune.json'))
+    mech = mechs('gri30')
+    n = 16384
+    p, f = _device_states(mech, n, False, 'F', seed=77)
+    for K, nv in ((PySpeciesRatesKernel, mech.NN), (PyJacobianKernel, mech.nnz)):
+        k = K(n, 0, mech=mech, conp=False, order='F', jac_format=1)
+        out = torch.empty(nv * n, dtype=torch.float64, device='cuda')
+        v, ms = k.autotune(p, f, out)
+        assert v in (1, 2, 11, 12, 14, 18) and ms > 0
+        assert (tmp_path / 'tune.json').exists()
+        k2 = K(n, 0, mech=mech, conp=False, order='F', jac_format=1)  # picks the tuned choice
+        out2 = torch.empty_like(out)
+        k2(p, f, out2)
+        torch.cuda.synchronize()
+        param = p.cpu().numpy()
+        phi = f.cpu().numpy().T.copy()
+        if K is PySpeciesRatesKernel:
+            ref, scale = rate_scales(mech, False, param, phi)
+            got = out2.view(-1, n).cpu().numpy().T
+            assert rate_error(got, ref, scale) <= 1.0
+        else:
+            m = 256
+            ref = oracle.to_layout(mech, oracle.jacobian(mech, False, param[:m], phi[:m]), 'F', 1)
+            got = out2.view(-1, n).cpu().numpy()[:, :m]
+            fro = np.linalg.norm(got - ref, axis=0) / np.linalg.norm(ref, axis=0)
+            assert fro.max() <= 1e-8

@@ -42,6 +42,7 @@ extern "C" {
 #define PYJAC_ERR_ALLOC 3        /* device or host allocation failed */
 #define PYJAC_ERR_UNSUPPORTED 4  /* option combination not supported */
 #define PYJAC_ERR_STATE 5        /* handle not ready (e.g. not resized) */
+#define PYJAC_ERR_IO 6           /* file missing, unreadable or too short */
 
 /* Arrhenius forms of simple_form / fall_form (rate_subs.py:100-124) */
 #define PYJAC_ARR_A 0          /* kf = A */
@@ -226,6 +227,23 @@ int pyjac_hip_batch_destroy(pyjac_batch *b);
 int pyjac_hip_autotune(pyjac_handle *h, int kernel, size_t n, const double *d_param,
                        const double *d_phi, double *d_out, void *stream, int *best_variant,
                        float *best_ms);
+
+/*
+ * Initial conditions on disk (read_initial_conditions,
+ * kernel_utils/common/read_initial_conditions.cpp.in:41-116): a binary file of
+ * num_states records of NN + 1 doubles, C order, each
+ * [T, param, V|P, n_0 .. n_{Ns-2}] (the extra variable is the state's V for
+ * CONP and P for CONV; param is the other one).  Fills host param[num_states]
+ * and host phi in the given order ('C': [n][NN], 'F': [NN][n]).  Where the
+ * reference prints and exits, this returns PYJAC_ERR_IO (missing file, fewer
+ * records than asked); a longer file is read up to num_states records, as the
+ * reference does.  nn is NN = Ns + 1 (the reference bakes it in).
+ */
+int pyjac_hip_read_initial_conditions(const char *filename, size_t num_states, int nn,
+                                      double *param, double *phi, char order);
+/* Raw dump of count doubles (write_data, kernel_utils/common/write_data.hpp:15-32),
+   the format the reference's validation outputs use; PYJAC_ERR_IO on failure. */
+int pyjac_hip_write_data(const char *filename, const double *arr, size_t count);
 
 #ifdef __cplusplus
 }

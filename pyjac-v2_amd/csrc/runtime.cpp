@@ -251,6 +251,47 @@ extern "C" {
 
 const char* pyjac_hip_last_error(void) { return g_last_error.c_str(); }
 
+// read_initial_conditions.cpp.in:41-116: one record per state, C order on disk
+int pyjac_hip_read_initial_conditions(const char* filename, size_t num_states, int nn,
+                                      double* param, double* phi, char order) {
+    if (!filename || nn < 2 || (num_states && (!param || !phi)) || (order != 'C' && order != 'F'))
+        return fail(PYJAC_ERR_ARG, "bad argument");
+    FILE* fp = std::fopen(filename, "rb");
+    if (!fp) return fail(PYJAC_ERR_IO, "could not open file: %s", filename);
+    std::vector<double> rec((size_t)nn + 1);
+    for (size_t i = 0; i < num_states; ++i) {
+        size_t got = std::fread(rec.data(), sizeof(double), rec.size(), fp);
+        if (got != rec.size()) {
+            std::fclose(fp);
+            return fail(PYJAC_ERR_IO,
+                        "file (%s) is incorrectly formatted: state %zu needs %d doubles, %zu read",
+                        filename, i, nn + 1, got);
+        }
+        param[i] = rec[1];
+        if (order == 'C') {
+            phi[i * nn] = rec[0];
+            for (int j = 1; j < nn; ++j) phi[i * nn + j] = rec[j + 1];
+        } else {
+            phi[i] = rec[0];
+            for (int j = 1; j < nn; ++j) phi[(size_t)j * num_states + i] = rec[j + 1];
+        }
+    }
+    std::fclose(fp);
+    return PYJAC_OK;
+}
+
+// write_data.hpp:15-32
+int pyjac_hip_write_data(const char* filename, const double* arr, size_t count) {
+    if (!filename || (count && !arr)) return fail(PYJAC_ERR_ARG, "bad argument");
+    FILE* fp = std::fopen(filename, "wb");
+    if (!fp) return fail(PYJAC_ERR_IO, "error opening file for data output: %s", filename);
+    size_t put = std::fwrite(arr, sizeof(double), count, fp);
+    int closed = std::fclose(fp);
+    if (put != count || closed != 0)
+        return fail(PYJAC_ERR_IO, "error writing %zu doubles to %s", count, filename);
+    return PYJAC_OK;
+}
+
 int pyjac_hip_create(const pyjac_mech_tables* mt, int conp, char order, int jac_format,
                      int jac_type, pyjac_handle** out) {
     if (!mt || !out) return fail(PYJAC_ERR_ARG, "null argument");

@@ -48,3 +48,17 @@ WORKLOADS = {
                                  jac_format=1, n=1048576,
                                  T_range=(1000.0, 2800.0), alpha=0.3),
 }
+
+
+def write_data_bin(path, param, phi):
+    """Write states as a ``data.bin`` initial-conditions file: one C-order
+    record ``[T, param, V|P, n_0 .. n_{Ns-2}]`` per state, the layout the
+    reference's tests write for its reader (tests/test_kernel_compilation.py
+    :212-217, tests/test_utils/__init__.py:1571-1576).  ``phi`` is
+    ``[n, NN]`` (C order)."""
+    phi = np.asarray(phi, dtype=np.float64).reshape(len(param), -1)
+    rec = np.concatenate((phi[:, :1], np.asarray(param, dtype=np.float64).reshape(-1, 1),
+                          phi[:, 1:]), axis=1)
+    with open(path, 'wb') as f:
+        np.ascontiguousarray(rec).tofile(f)
+    return path

@@ -65,6 +65,8 @@ def load_library(path=None):
     L.pyjac_hip_batch_destroy.argtypes = [P]
     L.pyjac_hip_autotune.argtypes = [P, I, S, P, P, P, P, ctypes.POINTER(I),
                                      ctypes.POINTER(ctypes.c_float)]
+    L.pyjac_hip_read_initial_conditions.argtypes = [ctypes.c_char_p, S, I, P, P, ctypes.c_char]
+    L.pyjac_hip_write_data.argtypes = [ctypes.c_char_p, P, S]
     if path == LIB_PATH or _lib is None:
         _lib = L
     return L
@@ -329,6 +331,42 @@ class PyChemUtilsKernel(_KernelBase):
         args = [_ptr(a)[0] for a in (phi, specific_energy, specific_heat, b)]
         check(self.lib.pyjac_hip_chem_utils(self._h, *args))
         return b, specific_energy, specific_heat
+
+
+def _host_f64(a, name):
+    if not isinstance(a, np.ndarray) or a.dtype != np.float64 or not (
+            a.flags['C_CONTIGUOUS'] or a.flags['F_CONTIGUOUS']) or not a.flags['WRITEABLE']:
+        raise TypeError('{} must be a writeable contiguous float64 numpy array'.format(name))
+
+
+def read_initial_conditions(filename, num, param, phi, order='C'):
+    """Fill ``param[num]`` and ``phi`` (``num * NN`` doubles in ``order``)
+    from a ``data.bin`` file of ``num`` records ``[T, param, V|P, n_0 ..
+    n_{Ns-2}]`` (read_initial_conditions.cpp.in:41-116; the Cython
+    ``read_ics`` of tests/test_utils/read_ic_wrapper.pyx, with the
+    reference's ``(param, phi)`` argument order).  Raises
+    ``HipRuntimeError`` where the reference exits."""
+    _host_f64(param, 'param')
+    _host_f64(phi, 'phi')
+    num = int(num)
+    if param.size < num or (num and phi.size % num) or phi.size < num * 2:
+        raise ValueError('param needs {0} entries and phi a multiple of {0}'.format(num))
+    nn = phi.size // num if num else 2
+    L = load_library()
+    check(L.pyjac_hip_read_initial_conditions(os.fsencode(filename), num, nn,
+                                              ctypes.c_void_p(param.ctypes.data),
+                                              ctypes.c_void_p(phi.ctypes.data),
+                                              order.encode()))
+    return param, phi
+
+
+def write_data(filename, arr):
+    """Raw dump of a float64 array in its memory order (write_data.hpp:15-32)."""
+    a = np.asarray(arr)
+    if a.dtype != np.float64 or not (a.flags['C_CONTIGUOUS'] or a.flags['F_CONTIGUOUS']):
+        raise TypeError('write_data takes a contiguous float64 array')
+    L = load_library()
+    check(L.pyjac_hip_write_data(os.fsencode(filename), ctypes.c_void_p(a.ctypes.data), a.size))
 
 
 # names under which the reference's generated module exposes the classes

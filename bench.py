@@ -37,6 +37,8 @@ def parse():
     ap.add_argument('--cpu-seconds', type=float, default=15.0,
                     help='approximate CPU work of the CPU-baseline sample')
     ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--no-surrogate', action='store_true')
+    ap.add_argument('--surrogate-states', type=int, default=262144)
     ap.add_argument('--copy-bytes', type=int, default=0,
                     help='run one device-to-device copy of this many bytes first (a kernel of '
                          'known traffic, to calibrate hardware byte counters against)')
@@ -85,6 +87,28 @@ def cpu_baseline(mech, budget_s):
     return {'value': n / dt, 'unit': 'Jacobians/s', 'cores': threads, 'kind': 'port',
             'sample': '{} GRI-3.0 CONV states, analytical Jacobian incl. species rates, '
                       'C++ restatement with OpenMP over states'.format(n)}
+
+
+SURROGATE_MODEL = os.path.join(ROOT, 'pyjac-v2_amd', 'pyjac', 'surrogate',
+                               'gri30_conv.safetensors')
+
+
+def surrogate_leg(mech, n):
+    """The species-rate surrogate with its error gate and FP64 fallback at the
+    1e-3 tolerance, against the FP64 species-rate kernel on n fresh states of
+    the same workload (F order): speed-up, accepted fraction, largest error."""
+    if not os.path.exists(SURROGATE_MODEL):
+        return None
+    from pyjac.surrogate import Surrogate
+    from pyjac.surrogate.train import evaluate
+    sur = Surrogate(mech, conp=False, order='F').load(SURROGATE_MODEL)
+    r = evaluate(sur, n, seed=4242, tol=1e-3)
+    return {'tol': 1e-3, 'states': n, 'speedup': r['speedup'],
+            'accepted_fraction': r['accepted_fraction'],
+            'max_error_all': r['max_error_all'],
+            'fp64_ms': r['fp64_ms'], 'surrogate_with_fallback_ms': r['surrogate_with_fallback_ms'],
+            'network_only_ms': r['network_only_ms'],
+            'error': 'max_k |w_k - w_ref_k| / max_k |w_ref_k| per state'}
 
 
 def main():
@@ -171,6 +195,10 @@ def main():
                      'kernel': 'jacobian_block_kernel',
                      'algorithmic_bytes_per_state': bytes_per_state},
     }
+    if not args.no_surrogate:
+        del d_jac, d_dphi
+        torch.cuda.empty_cache()
+        line['surrogate'] = surrogate_leg(mech, args.surrogate_states)
     if not args.no_cpu:
         line['cpu_baseline'] = cpu_baseline(mech, args.cpu_seconds)
     print(json.dumps(line))

@@ -196,7 +196,11 @@ size_t pyjac_hip_required_memory(const pyjac_handle *h);
    generated code fixes the layout at generation time (--width/--depth).
    kernel: 0 species rates, 1 Jacobian, 2 chem utils.  variant: 0 automatic
    (the tuned choice), 1 lane-per-state kernels, 2 wave-per-state kernels,
-   10 + W block-per-state kernels with W = 1, 2, 4 or 8 waves per state. */
+   10 + W block-per-state kernels with W = 1, 2, 4 or 8 waves per state,
+   20 + W (Jacobian) block-per-state kernels that gather the Jacobian in
+   device memory (the automatic choice when the dense accumulator does not
+   fit in LDS).  PYJAC_ERR_UNSUPPORTED when this mechanism cannot run the
+   variant (its arrays exceed the variant's LDS). */
 int pyjac_hip_set_variant(pyjac_handle *h, int kernel, int variant);
 int pyjac_hip_get_variant(const pyjac_handle *h, int kernel);
 

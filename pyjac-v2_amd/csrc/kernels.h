@@ -56,6 +56,16 @@ struct BlockTables {
     const void* combo;       // int4 [n_combo]: pair, dense index, efficiency (2 ints)
     const void* out_pos;     // int2 [jac_nnz]: dense index, row of the bath-gas sum or -1
     int n_pairs, n_combo;
+    // variant gathering in device memory (dense Jacobian larger than LDS)
+    const int* acc_map;      // [NN * NN] dense index -> output entry, -1 if not stored
+    const int* rec_pos;      // [Nr][8][WAVE_MAXSP]: per record, the contribution slot of
+                             // (column slot 0-5 / T / extra variable, row slot), -1 for none
+    const int* ent_start;    // [jac_nnz + 1]: contribution slots of each output entry
+    const int* contrib_ent;  // [n_contrib]: output entry of each contribution slot
+    const int* ent_combo;    // [jac_nnz + 1]: third-body columns of each output entry
+    const void* gcombo;      // int4 [n_combo]: pair, output entry, efficiency (by entry)
+    const int* ent_rc;       // [jac_nnz]: (row << 16) | column of each output entry
+    int n_contrib;
 };
 
 // LDS offsets (in doubles) of one workgroup's arrays
@@ -73,8 +83,9 @@ struct BlockArgs {
     BlockLds L;
 };
 
-BlockLds block_lds(const DevMech& m, bool jac, int n_pairs);
-size_t block_lds_bytes(const DevMech& m, bool jac, int n_pairs);
+// dense: the Jacobian accumulator in LDS (false: the device-memory variant)
+BlockLds block_lds(const DevMech& m, bool jac, int n_pairs, bool dense = true);
+size_t block_lds_bytes(const DevMech& m, bool jac, int n_pairs, bool dense = true);
 
 // one workgroup of 64 * wps threads per state
 hipError_t launch_rates_block(const DevMech& m, const BlockArgs* args, const BlockLds& L, int wps,
@@ -84,6 +95,15 @@ hipError_t launch_rates_block(const DevMech& m, const BlockArgs* args, const Blo
 hipError_t launch_jacobian_block(const DevMech& m, const BlockArgs* args, const BlockLds& L,
                                  int wps, int conp, char order, long n, const double* param,
                                  const double* phi, double* jac, hipStream_t stream);
+
+// one workgroup of 64 * wps threads per state, Jacobian gathered in device
+// memory, chunk_states states (a multiple of 64) per launch: chunk_buf holds
+// their contributions (n_contrib each) and, in F order, their entries, which
+// a tiled transpose moves into the output; C order writes entries in place
+hipError_t launch_jacobian_global(const DevMech& m, const BlockArgs* args, const BlockLds& L,
+                                  int wps, int conp, char order, long n, const double* param,
+                                  const double* phi, double* jac, double* chunk_buf,
+                                  long chunk_states, size_t n_contrib, hipStream_t stream);
 
 // ---- ragged batch: several handles' states in one block-kernel launch
 constexpr int RAGGED_MAX = 8;

@@ -31,6 +31,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+
 #include "device_mech.h"
 #include "kernels.h"
 
@@ -656,20 +658,46 @@ __device__ __forceinline__ void scatter_wdot(Ctx& c, const Rec& x, double net, i
     }
 }
 
+// Where a reaction's Jacobian contributions go: added into a dense NN x NN
+// array in LDS (mechanisms whose state fits the CU's LDS), or stored, one
+// slot each, into the state's contribution buffer in device memory (larger
+// mechanisms).  The host orders the buffer's slots by output entry, so a
+// later gather sums each entry's contiguous run: no atomics, and a fixed
+// summation order.
+struct JacAcc {
+    lds_double* lds;
+    double* g;
+};
+
+// one contribution: to the dense LDS entry, or to the buffer slot that the
+// record's slot table holds for (column slot, row slot)
+template <bool G>
+__device__ __forceinline__ void jput(const JacAcc& a, const int* __restrict__ pos, int slot,
+                                     int dense, double v) {
+    if constexpr (!G) {
+        lds_add(a.lds + dense, v);
+    } else {
+        const int e = pos[slot];
+        if (e >= 0) a.g[e] = v;
+    }
+}
+
 // all per-reaction Jacobian contributions of one reaction: dRopidnj(+_ns)
 // (create_jacobian.py:4169-4484), dRopi*_dT(+_ns) (:2978-2992 and the _ns
 // kernels), dRopi*_dE(+_ns) (:1040-1080, :1158-1300) and the third-body
 // parts through the per-entry factors of phase 2, scattered into LDS.
 // Slots that contribute nothing add 0 to an entry of the reaction's own
 // row instead of branching.
+template <bool G>
 __device__ __forceinline__ void reaction_jacobian(Ctx& c, const State& st, const Rec& x,
-                                                  const Rop& v, double ropi, int gf) {
+                                                  const Rop& v, double ropi, int gf,
+                                                  const JacAcc& acc,
+                                                  const int* __restrict__ ent) {
     const int ns = c.ns, NN = c.NN;
     const int cm = gf & GF_CNT;
     const int conp = c.conp;
     const double Tinv = st.Traw_inv;
     const double kr_i = v.kr;
-    lds_double* J = c.J();
     // rows: species with a net coefficient other than the bath gas;
     // columns: species other than the bath gas
     int roff[MAXSP], coff[MAXSP];
@@ -721,7 +749,7 @@ __device__ __forceinline__ void reaction_jacobian(Ctx& c, const State& st, const
 #pragma unroll
             for (int k = 0; k < MAXSP; ++k) {
                 if (k >= cm) break;
-                lds_add(J + roff[k] + coff[j], nuk[k] * d);
+                jput<G>(acc, ent, j * MAXSP + k, roff[k] + coff[j], nuk[k] * d);
             }
         }
     }
@@ -791,8 +819,8 @@ __device__ __forceinline__ void reaction_jacobian(Ctx& c, const State& st, const
 #pragma unroll
     for (int k = 0; k < MAXSP; ++k) {
         if (k >= cm) break;
-        lds_add(J + roff[k], nuk[k] * dT);
-        lds_add(J + roff[k] + 1, nuk[k] * dE);
+        jput<G>(acc, ent, 6 * MAXSP + k, roff[k], nuk[k] * dT);
+        jput<G>(acc, ent, 7 * MAXSP + k, roff[k] + 1, nuk[k] * dE);
     }
     if (gf & (GF_NS | GF_THD)) {
 #pragma unroll
@@ -913,6 +941,7 @@ __device__ __forceinline__ void jacobian_state(const BlockArgs* __restrict__ A, 
     Ctx c = make_ctx(A, smem, conp);
     const int Ns = c.Ns, NN = c.NN, ns = c.ns, Nr = c.m.Nr, nout = c.m.jac_nnz;
     lds_double* J = c.J();
+    const JacAcc acc{J, nullptr};
     if (tid == 0) { c.conc(Ns) = 1.0; c.b(Ns) = 0.0; c.db(Ns) = 0.0; }
     for (int k = tid; k < Ns - 1; k += NT) c.mw(k) = c.m.mw_post[k];
     const double* ps = phi + s * L.sN;
@@ -941,7 +970,7 @@ __device__ __forceinline__ void jacobian_state(const BlockArgs* __restrict__ A, 
             const Rop v = rates_of_progress(c, st, x, gf);
             const double ropi = v.rf - v.rr;
             scatter_wdot(c, x, ropi * v.ci, gf);
-            reaction_jacobian(c, st, x, v, ropi, gf);
+            reaction_jacobian<false>(c, st, x, v, ropi, gf, acc, nullptr);
         }
     }
     __syncthreads();
@@ -1088,6 +1117,216 @@ __device__ __forceinline__ void jacobian_state(const BlockArgs* __restrict__ A, 
     }
 }
 
+// entries per gather tile of the device-memory variant (LDS doubles)
+constexpr int GTILE = 2048;
+
+// Agent-coherent load: the data was stored by other waves of the workgroup
+// in this launch, so the read must not be served by a stale L1 line.
+__device__ __forceinline__ double coh_load(const double* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Every memory operation of this wave completed before the barrier that
+// follows, so that the next phase of the workgroup sees it.
+__device__ __forceinline__ void wait_memory() { __builtin_amdgcn_s_waitcnt(0); }
+
+// Analytical Jacobian of state s for mechanisms whose dense accumulator does
+// not fit in LDS.  The species-level arrays stay in LDS; each reaction stores
+// its contributions into the state's buffer cb (one slot per contribution,
+// slots grouped by output entry); a gather then sums every stored entry's
+// run, adds its third-body columns and bath-gas sum, and writes the entry to
+// out (the state's nout entries in output order).  Same arithmetic as
+// jacobian_state; the rows of T and the extra variable sum the stored
+// species entries of each column plus the bath-gas sums of every row.
+template <int WPS>
+__device__ __forceinline__ void jacobian_global_state(const BlockArgs* __restrict__ A, int conp,
+                                                      long s, const double* __restrict__ param,
+                                                      const double* __restrict__ phi,
+                                                      double* __restrict__ cb,
+                                                      double* __restrict__ out, Layout L,
+                                                      double* smem) {
+    constexpr int NT = 64 * WPS;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    Ctx c = make_ctx(A, smem, conp);
+    const int Ns = c.Ns, NN = c.NN, ns = c.ns, Nr = c.m.Nr, nout = c.m.jac_nnz;
+    const JacAcc acc{nullptr, cb};
+    lds_double* cs0 = c.sm + c.L.jac;  // column sums of the T row
+    lds_double* cs1 = cs0 + NN;        // and of the extra-variable row
+    lds_double* col0 = cs1 + NN;       // columns 0 and 1 of the species rows
+    lds_double* col1 = col0 + NN;
+    if (tid == 0) { c.conc(Ns) = 1.0; c.b(Ns) = 0.0; c.db(Ns) = 0.0; }
+    for (int k = tid; k < NN; k += NT) {
+        if (k < Ns - 1) c.mw(k) = c.m.mw_post[k];
+        cs0[k] = 0.0;
+        cs1[k] = 0.0;
+        col0[k] = 0.0;
+        col1[k] = 0.0;
+    }
+    for (int p = tid; p < c.bt.n_pairs; p += NT) c.Cp(p) = 0.0;
+    const double* ps = phi + s * L.sN;
+    auto ph = [&](int i) { return ps[i * L.sI]; };
+    const State st = load_state(conp, param[s], ph(0), ph(1));
+    phase_species<true, NT>(c, st, tid, ph);
+    __syncthreads();
+    bath_gas(c, st, lane);
+    phase_pdep<true, WPS>(c, st, wave, lane);
+    __syncthreads();
+    // phase 3: reactions
+    for (int pass = 0; pass * NT < Nr; ++pass) {
+        const int r = pass * NT + tid;
+        const int gf = sld(c.bt.grp_flags, pass * WPS + wave);
+        if (r < Nr) {
+            const Rec x = load_rec(c.bt, r);
+            const Rop v = rates_of_progress(c, st, x, gf);
+            const double ropi = v.rf - v.rr;
+            scatter_wdot(c, x, ropi * v.ci, gf);
+            reaction_jacobian<true>(c, st, x, v, ropi, gf, acc,
+                                    c.bt.rec_pos + (size_t)r * (8 * MAXSP));
+        }
+    }
+    wait_memory();
+    __syncthreads();
+    double heat_tot, Tdot;
+    {
+        double lo = 0.0, up = 0.0;
+        for (int k = lane; k < Ns; k += 64) {
+            lo += c.conc(k) * c.heat(k);
+            up += c.energy(k) * c.wdot(k);
+        }
+        heat_tot = wave_sum(lo);
+        Tdot = 0.0 + -1.0 * wave_sum(up) / heat_tot;
+    }
+    const double T = st.T, Tinv = st.Traw_inv;
+    const double fixed = conp ? st.P : st.V;
+    const double extra = conp ? st.V : st.P;
+    const double e_ns = c.energy(ns), h_ns = c.heat(ns);
+    const double inv_vh = 1.0 / (st.V * heat_tot);
+    // phase 4: gather the stored species-row entries, GTILE entries at a
+    // time: the contributions of a tile's entries are contiguous, so the
+    // threads stride over them with coalesced loads and add each into its
+    // entry's LDS slot; then each entry of the tile is finished
+    {
+        const int* es = c.bt.ent_start;
+        const int* ce = c.bt.contrib_ent;
+        const int* ec = c.bt.ent_combo;
+        const int* rc = c.bt.ent_rc;
+        const int4* cmb = (const int4*)c.bt.gcombo;
+        lds_double* tile = col1 + NN;
+        for (int t0 = 0; t0 < nout; t0 += GTILE) {
+            const int t1 = min(nout, t0 + GTILE);
+            for (int i = tid; i < t1 - t0; i += NT) tile[i] = 0.0;
+            __syncthreads();
+            const int q0 = es[t0], q1 = es[t1];
+            for (int q = q0 + tid; q < q1; q += NT) lds_add(tile + (ce[q] - t0), coh_load(cb + q));
+            __syncthreads();
+            for (int e = t0 + tid; e < t1; e += NT) {
+                const int t = rc[e];
+                const int row = t >> 16, col = t & 0xffff;
+                if (row < 2) continue;  // rows 0 and 1 come last
+                double v = tile[e - t0];
+                for (int q = ec[e], q1 = ec[e + 1]; q < q1; ++q) {
+                    const int4 m4 = cmb[q];
+                    v += c.Cp(m4.x) * __hiloint2double(m4.w, m4.z);
+                }
+                const int k = row - 2;
+                if (col >= 2) {
+                    const double mw = c.mw(k);
+                    lds_add(cs0 + col - 2, (c.energy(k) + -1.0 * e_ns * mw) * v);
+                    lds_add(cs1 + col - 2, (1.0 + -1.0 * mw) * v);
+                    v += c.U(k);
+                } else if (col == 0) {
+                    col0[k] = v;
+                } else {
+                    col1[k] = v;
+                }
+                out[e] = v;
+            }
+            __syncthreads();
+        }
+    }
+    // the bath-gas sums join every species column of their row
+    double su0 = 0.0, su1 = 0.0;
+    for (int k = lane; k < Ns - 1; k += 64) {
+        const double u = c.U(k), mw = c.mw(k);
+        su0 += (c.energy(k) + -1.0 * e_ns * mw) * u;
+        su1 += (1.0 + -1.0 * mw) * u;
+    }
+    su0 = wave_sum(su0);
+    su1 = wave_sum(su1);
+    __syncthreads();
+    // phase 5: rows 0 and 1 of the species columns (dTdot_dnj, dEdot_dnj,
+    // create_jacobian.py:3266-3484)
+    const int* amap = c.bt.acc_map;
+    for (int j = tid; j < Ns - 1; j += NT) {
+        double j0 = -1.0 * (cs0[j] + su0) * inv_vh;
+        j0 = j0 + -1.0 * Tdot * (c.heat(j) + -1.0 * h_ns) * inv_vh;
+        const double j1 = T * RU * (cs1[j] + su1) / fixed + extra * j0 * Tinv;
+        out[amap[j + 2]] = j0;
+        out[amap[NN + j + 2]] = j1;
+    }
+    if (wave == WPS - 1) {
+        // columns 0 and 1 of rows 0 and 1 (create_jacobian.py:1595-2114)
+        const double Vinv = 1.0 / st.V;
+        double rate_sum = 0.0, sumT = 0.0, dTsumE = 0.0, specsum = 0.0, sumE = 0.0,
+               sumEE = 0.0, dTsum = 0.0;
+        for (int i = lane; i < Ns - 1; i += 64) {
+            const double j0 = col0[i];
+            const double j1 = col1[i];
+            const double mw = c.mw(i);
+            const double w = c.wdot(i);
+            rate_sum += Vinv * j0 * (-1.0 * c.energy(i) + e_ns * mw) +
+                        w * (-1.0 * c.heat(i) + mw * h_ns);
+            dTsum += (h_ns * Tinv + -1.0 * c.dheat(i)) * c.conc(i);
+            if (conp) {
+                sumT += (1.0 + -1.0 * mw) * (Vinv * j0 + Tinv * w);
+                dTsumE += (c.heat(i) + -1.0 * h_ns) * c.conc(i);
+                specsum += (c.energy(i) + -1.0 * e_ns * mw) * (j1 + -1.0 * w);
+            } else {
+                sumT += (1.0 - mw) * (j0 * (T / st.V) + w);
+                sumE += (c.energy(i) - e_ns * mw) * j1;
+            }
+            sumEE += (1 - mw) * j1;
+        }
+        rate_sum = wave_sum(rate_sum);
+        dTsum = wave_sum(dTsum) + (h_ns * Tinv + -1.0 * c.dheat(ns)) * c.conc(ns);
+        sumT = wave_sum(sumT);
+        sumEE = wave_sum(sumEE);
+        if (conp) {
+            dTsumE = wave_sum(dTsumE);
+            specsum = wave_sum(specsum);
+        } else {
+            sumE = wave_sum(sumE);
+        }
+        if (lane == 0) {
+            const double j00 = (Tdot * dTsum + rate_sum) / heat_tot;
+            out[amap[0]] = j00;
+            double j10;
+            if (conp) {
+                j10 = st.V * Tinv * (j00 + -1.0 * Tinv * Tdot);
+                j10 = j10 + RU * T * st.V * sumT / st.P;
+            } else {
+                j10 = st.P * (j00 - Tdot * Tinv) * Tinv;
+                j10 = j10 + RU * sumT;
+            }
+            out[amap[NN]] = j10;
+            double j01;
+            if (conp) {
+                j01 = (Tdot * dTsumE + -1.0 * specsum) * (1.0 / (heat_tot * st.V));
+            } else {
+                const double spec_inv = 1.0 / heat_tot;
+                j01 = -(Tdot * h_ns / (RU * T)) * spec_inv;
+                j01 = j01 - (sumE * Vinv) * spec_inv;
+            }
+            out[amap[1]] = j01;
+            const double var = conp ? st.V : st.P;
+            const double par = conp ? st.P : st.V;
+            const double j11 = (var * j01 + Tdot) / T;
+            out[amap[NN + 1]] = j11 + RU * T * sumEE / par;
+        }
+    }
+}
+
 template <int WPS>
 __global__ void __launch_bounds__(64 * WPS)
 rates_block_kernel(const BlockArgs* __restrict__ A, int conp, long n,
@@ -1108,6 +1347,45 @@ jacobian_block_kernel(const BlockArgs* __restrict__ A, int conp, long n,
     const long s = state_of_block(blockIdx.x, n);
     if (s >= n) return;
     jacobian_state<WPS>(A, conp, n, s, param, phi, jac, L, out_C, smem);
+}
+
+// states s0 .. s0 + count - 1, one workgroup each; state s0 + i stores its
+// contributions at cbuf + i * ncontrib and its entries at out + i * nout (the
+// output itself in C order, a chunk buffer in F order)
+template <int WPS>
+__global__ void __launch_bounds__(64 * WPS)
+jacobian_global_kernel(const BlockArgs* __restrict__ A, int conp, long s0, long count,
+                       const double* __restrict__ param, const double* __restrict__ phi,
+                       double* __restrict__ cbuf, double* __restrict__ out, Layout L) {
+    extern __shared__ double smem[];
+    const long i = blockIdx.x;
+    if (i >= count) return;
+    jacobian_global_state<WPS>(A, conp, s0 + i, param, phi,
+                               cbuf + i * (size_t)A->bt.n_contrib,
+                               out + i * (size_t)A->m.jac_nnz, L, smem);
+}
+
+// chunk buffer [count][nout] (entries of a state contiguous) -> F-order
+// output [nout][n], states s0 .. s0 + count - 1; 64 x 64 tiles through LDS so
+// that reads and writes are both whole lines
+__global__ void __launch_bounds__(256)
+entries_to_f_kernel(const double* __restrict__ src, long count, int nout,
+                    double* __restrict__ dst, long n, long s0) {
+    __shared__ double tile[64][65];
+    const long t0 = (long)blockIdx.x * 64;
+    const int e0 = blockIdx.y * 64;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    for (int r = ty; r < 64; r += 4) {
+        const long st = t0 + r;
+        const int e = e0 + tx;
+        if (st < count && e < nout) tile[r][tx] = src[st * nout + e];
+    }
+    __syncthreads();
+    for (int r = ty; r < 64; r += 4) {
+        const int e = e0 + r;
+        const long st = t0 + tx;
+        if (st < count && e < nout) dst[(size_t)e * n + s0 + st] = tile[tx][r];
+    }
 }
 
 // Ragged batch: states of several mechanisms / option sets in one launch.
@@ -1157,7 +1435,7 @@ __global__ void __launch_bounds__(64 * WPS) jacobian_ragged_kernel(RaggedArgs R)
 
 namespace pjk_host {
 
-BlockLds block_lds(const DevMech& m, bool jac, int bt_pairs) {
+BlockLds block_lds(const DevMech& m, bool jac, int bt_pairs, bool dense) {
     BlockLds L{};
     int o = 0;
     const int Ns = m.Ns;
@@ -1186,14 +1464,17 @@ BlockLds block_lds(const DevMech& m, bool jac, int bt_pairs) {
         L.dkT = take(m.n_plog + m.n_cheb);
         L.dkP = take(m.n_plog + m.n_cheb);
         o = (o + 1) & ~1;  // 16-byte alignment of the accumulator
-        L.jac = take(m.NN * m.NN);
+        // dense accumulator, or (variant gathering in device memory) the
+        // column sums of rows 0 and 1, columns 0 and 1 of the species rows
+        // and the gather tile
+        L.jac = take(dense ? m.NN * m.NN : 4 * m.NN + pjb::GTILE);
     }
     L.total = o;
     return L;
 }
 
-size_t block_lds_bytes(const DevMech& m, bool jac, int n_pairs) {
-    return (size_t)block_lds(m, jac, n_pairs).total * sizeof(double);
+size_t block_lds_bytes(const DevMech& m, bool jac, int n_pairs, bool dense) {
+    return (size_t)block_lds(m, jac, n_pairs, dense).total * sizeof(double);
 }
 
 template <class K>
@@ -1253,6 +1534,48 @@ hipError_t launch_jacobian_block(const DevMech& m, const BlockArgs* args, const 
     }
 #undef PJB_JAC
     return hipGetLastError();
+}
+
+hipError_t launch_jacobian_global(const DevMech& m, const BlockArgs* args, const BlockLds& BL,
+                                  int wps, int conp, char order, long n, const double* param,
+                                  const double* phi, double* jac, double* chunk_buf,
+                                  long chunk_states, size_t n_contrib, hipStream_t stream) {
+    pjb::Layout L = order == 'F' ? pjb::Layout{1, (size_t)n} : pjb::Layout{(size_t)m.NN, 1};
+    size_t lds = (size_t)BL.total * sizeof(double);
+    const int nout = m.jac_nnz;
+    if (!chunk_buf || chunk_states < 64 || chunk_states % 64) return hipErrorInvalidValue;
+    // chunk buffer: contributions of chunk_states states, then (F order)
+    // their entries
+    double* cbuf = chunk_buf;
+    double* ebuf = chunk_buf + (size_t)chunk_states * n_contrib;
+    for (long s0 = 0; s0 < n; s0 += chunk_states) {
+        const long cnt = std::min(chunk_states, n - s0);
+        double* out = order == 'F' ? ebuf : jac + (size_t)s0 * nout;
+#define PJB_GJ(W)                                                                         \
+    case W: {                                                                             \
+        hipError_t e = set_lds(pjb::jacobian_global_kernel<W>, lds);                     \
+        if (e != hipSuccess) return e;                                                    \
+        hipLaunchKernelGGL(pjb::jacobian_global_kernel<W>, dim3(cnt), dim3(64 * W), lds,  \
+                           stream, args, conp, s0, cnt, param, phi, cbuf, out, L);        \
+        break;                                                                            \
+    }
+        switch (wps) {
+            PJB_GJ(1)
+            PJB_GJ(2)
+            PJB_GJ(4)
+            PJB_GJ(8)
+            default: return hipErrorInvalidValue;
+        }
+#undef PJB_GJ
+        if (order == 'F') {
+            dim3 grid((unsigned)((cnt + 63) / 64), (unsigned)((nout + 63) / 64));
+            hipLaunchKernelGGL(pjb::entries_to_f_kernel, grid, dim3(256), 0, stream,
+                               (const double*)ebuf, cnt, nout, jac, n, s0);
+        }
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 hipError_t launch_ragged(int kernel, int wps, int count, const RaggedLaunch* members,

@@ -68,6 +68,13 @@ struct pyjac_handle {
     pjk_host::BlockTables bt{};
     bool wave_ok_rates = false, wave_ok_jac = false;
     bool block_ok_rates = false, block_ok_jac = false;
+    // Jacobian accumulated in device memory (dense accumulator beyond LDS)
+    bool glob_ok_jac = false;
+    pjk_host::BlockArgs* d_args_g = nullptr;
+    pjk_host::BlockLds lds_g{};
+    int wps_g = 0;
+    double* d_chunk = nullptr;  // F order: one chunk of states' entries
+    long chunk_states = 0;
     // block kernels: device copies of their arguments, LDS layouts and the
     // tuned number of waves per state (0: default)
     pjk_host::BlockArgs* d_args[2] = {nullptr, nullptr};
@@ -134,7 +141,8 @@ int wave_grid(const pyjac_handle* h, size_t lds_bytes, long n) {
 
 // Launch shape of the block-per-state kernels.  Variants: 0 automatic,
 // 1 lane-per-state, 2 wave-per-state, 10 + W block-per-state with W waves
-// per state (W = 1, 2, 4, 8).
+// per state (W = 1, 2, 4, 8), 20 + W block-per-state Jacobian accumulated in
+// device memory.
 constexpr size_t LDS_PER_CU = 160 * 1024;
 
 int block_wps(const pyjac_handle* h, int kernel) {
@@ -147,12 +155,48 @@ int block_wps(const pyjac_handle* h, int kernel) {
 bool use_block(const pyjac_handle* h, int kernel) {
     int v = h->variant[kernel];
     bool ok = kernel == 0 ? h->block_ok_rates : h->block_ok_jac;
-    return ok && (v == 0 || v > 10);
+    return ok && (v == 0 || (v > 10 && v < 20));
+}
+
+// the Jacobian accumulated in device memory: asked for, or automatic when
+// the dense accumulator does not fit in LDS
+bool use_glob(const pyjac_handle* h) {
+    int v = h->variant[1];
+    if (!h->glob_ok_jac) return false;
+    return (v > 20 && v < 30) || (v == 0 && !h->block_ok_jac);
+}
+
+int glob_wps(const pyjac_handle* h) {
+    int v = h->variant[1];
+    if (v > 20) return v - 20;
+    if (h->wps_g) return h->wps_g;
+    // measured on MI355X: 8 waves for 3000 reactions, 4 for 900, 2 for 325
+    const int Nr = h->m.Nr;
+    return Nr >= 2000 ? 8 : (Nr >= 600 ? 4 : 2);
+}
+
+// chunk buffer of the device-memory variant (about 2 GiB): per state its
+// contributions and, in F order, its entries
+int ensure_chunk(pyjac_handle* h, long n) {
+    const size_t per = ((size_t)h->bt.n_contrib + (h->order == 'F' ? h->m.jac_nnz : 0)) *
+                       sizeof(double);
+    long want = (long)(((size_t)2 << 30) / std::max<size_t>(per, 1)) / 64 * 64;
+    const char* env = getenv("PYJAC_HIP_CHUNK_STATES");  // override, for tests of the chunking
+    if (env && atol(env) > 0) want = atol(env) / 64 * 64;
+    want = std::max(64L, std::min(want, (n + 63) / 64 * 64));
+    if (h->d_chunk && h->chunk_states >= want) return PYJAC_OK;
+    (void)hipFree(h->d_chunk);
+    h->d_chunk = nullptr;
+    h->chunk_states = 0;
+    HIP_TRY(hipMalloc(&h->d_chunk, (size_t)want * per));
+    h->chunk_states = want;
+    return PYJAC_OK;
 }
 
 // workspace of the lane-per-state kernels for n states, allocated ahead of
 // a graph capture (no allocation may happen while a stream is captured)
 int prepare_lane(pyjac_handle* h, int kernel, long n) {
+    if (kernel == 1 && use_glob(h)) return ensure_chunk(h, n);
     size_t per = pjk_host::ws_doubles_per_state(h->m, kernel == 1);
     int grid = pick_grid(h, per, h->variant[kernel], n);
     return ensure_ws(h, per * (size_t)grid * h->block * sizeof(double));
@@ -187,6 +231,15 @@ int run_jac(pyjac_handle* h, long n, const double* param, const double* phi, dou
     if (use_block(h, 1)) {
         HIP_TRY(pjk_host::launch_jacobian_block(h->m, h->d_args[1], h->lds[1], block_wps(h, 1),
                                                 h->conp, h->order, n, param, phi, jac, s));
+        return PYJAC_OK;
+    }
+    if (use_glob(h)) {
+        int rc = ensure_chunk(h, n);
+        if (rc) return rc;
+        HIP_TRY(pjk_host::launch_jacobian_global(h->m, h->d_args_g, h->lds_g, glob_wps(h),
+                                                 h->conp, h->order, n, param, phi, jac,
+                                                 h->d_chunk, h->chunk_states,
+                                                 (size_t)h->bt.n_contrib, s));
         return PYJAC_OK;
     }
     if (h->wave_ok_jac && h->variant[1] == 2) {
@@ -486,6 +539,69 @@ int pyjac_hip_create(const pyjac_mech_tables* mt, int conp, char order, int jac_
         out_pos[2 * e] = p;
         out_pos[2 * e + 1] = (row >= 2 && col >= 2) ? row - 2 : -1;
     }
+    // tables of the variant gathering in device memory: dense index ->
+    // output entry; every reaction's contribution slots, grouped by output
+    // entry (ent_start) in record order, and each record's slot table; the
+    // third-body columns sorted by output entry (ent_combo); each entry's
+    // row and column
+    std::vector<int> acc_map((size_t)NN * NN, -1);
+    for (size_t e = 0; e < jac_pos.size(); ++e) acc_map[jac_pos[e]] = (int)e;
+    const size_t nout_ = jac_pos.size();
+    bool glob_tables = NN <= 0x7fff;
+    // rows 0 and 1 are stored in full (the kernel writes them by entry)
+    for (int j = 0; j < NN; ++j) glob_tables = glob_tables && acc_map[j] >= 0 && acc_map[NN + j] >= 0;
+    const int MS = pjk_host::WAVE_MAXSP;
+    std::vector<int> rec_pos((size_t)Nr * 8 * MS, -1);
+    std::vector<long> ent_count(nout_ + 1, 0);
+    for (int pass = 0; pass < 2 && packable; ++pass) {
+        // pass 0 counts the contributions of each entry, pass 1 places them
+        std::vector<long> next;
+        if (pass == 1) next.assign(ent_count.begin(), ent_count.end() - 1);
+        for (int q = 0; q < Nr; ++q) {
+            const int r = perm[q], e0 = mt->rxn_off[r], cnt = mt->rxn_off[r + 1] - e0;
+            for (int k = 0; k < cnt; ++k) {
+                const int sk = mt->rxn_spec[e0 + k];
+                if (sk == Ns - 1 || mt->nu_prod[e0 + k] == mt->nu_reac[e0 + k]) continue;
+                const size_t rowp = (size_t)(sk + 2) * NN;
+                for (int j = 0; j < 8; ++j) {
+                    int dense;
+                    if (j < MS) {
+                        if (j >= cnt || mt->rxn_spec[e0 + j] == Ns - 1) continue;
+                        dense = (int)rowp + mt->rxn_spec[e0 + j] + 2;
+                    } else {
+                        dense = (int)rowp + (j - MS);
+                    }
+                    // an unstored pair is a structural zero (e.g. the product
+                    // columns of an irreversible reaction): nothing to store
+                    const int e = acc_map[dense];
+                    if (e < 0) continue;
+                    if (pass == 0) ++ent_count[e + 1];
+                    else rec_pos[((size_t)q * 8 + j) * MS + k] = (int)next[e]++;
+                }
+            }
+        }
+        if (pass == 0)
+            for (size_t e = 0; e < nout_; ++e) ent_count[e + 1] += ent_count[e];
+    }
+    const long n_contrib = ent_count[nout_];
+    glob_tables = glob_tables && n_contrib < (1L << 31);
+    std::vector<int> ent_start(ent_count.begin(), ent_count.end());
+    std::vector<int> contrib_ent(std::max<long>(n_contrib, 1), 0);
+    for (size_t e = 0; e < nout_; ++e)
+        for (long q = ent_count[e]; q < ent_count[e + 1]; ++q) contrib_ent[q] = (int)e;
+    std::vector<Combo> gcombos(combos);
+    for (auto& cb : gcombos) {
+        cb.dst = acc_map[cb.dst];
+        glob_tables = glob_tables && cb.dst >= 0;
+    }
+    std::stable_sort(gcombos.begin(), gcombos.end(),
+                     [](const Combo& a, const Combo& b) { return a.dst < b.dst; });
+    std::vector<int> ent_combo(nout_ + 1, 0);
+    for (const auto& cb : gcombos)
+        if (cb.dst >= 0) ++ent_combo[cb.dst + 1];
+    for (size_t e = 0; e < nout_; ++e) ent_combo[e + 1] += ent_combo[e];
+    std::vector<int> ent_rc(nout_);
+    for (size_t e = 0; e < nout_; ++e) ent_rc[e] = ((jac_pos[e] / NN) << 16) | (jac_pos[e] % NN);
 
     Packer P;
     DevMech& m = h->m;
@@ -571,6 +687,13 @@ int pyjac_hip_create(const pyjac_mech_tables* mt, int conp, char order, int jac_
     size_t off_thd_pairs = P.add(thd_pairs.data(), thd_pairs.size());
     size_t off_combo = P.add(combos.data(), combos.size());
     size_t off_out_pos = P.add(out_pos.data(), out_pos.size());
+    size_t off_acc_map = P.add(acc_map.data(), acc_map.size());
+    size_t off_rec_pos = P.add(rec_pos.data(), rec_pos.size());
+    size_t off_ent_start = P.add(ent_start.data(), ent_start.size());
+    size_t off_contrib_ent = P.add(contrib_ent.data(), contrib_ent.size());
+    size_t off_ent_combo = P.add(ent_combo.data(), ent_combo.size());
+    size_t off_gcombo = P.add(gcombos.data(), gcombos.size());
+    size_t off_ent_rc = P.add(ent_rc.data(), ent_rc.size());
     hipError_t e = hipMalloc(&h->d_blob, P.bytes.size());
     if (e == hipSuccess)
         e = hipMemcpy(h->d_blob, P.bytes.data(), P.bytes.size(), hipMemcpyHostToDevice);
@@ -595,6 +718,14 @@ int pyjac_hip_create(const pyjac_mech_tables* mt, int conp, char order, int jac_
     h->bt.n_pairs = (int)pairs.size();
     h->bt.n_combo = (int)combos.size();
     h->bt.out_pos = h->d_blob + off_out_pos;
+    h->bt.acc_map = (const int*)(h->d_blob + off_acc_map);
+    h->bt.rec_pos = (const int*)(h->d_blob + off_rec_pos);
+    h->bt.ent_start = (const int*)(h->d_blob + off_ent_start);
+    h->bt.contrib_ent = (const int*)(h->d_blob + off_contrib_ent);
+    h->bt.ent_combo = (const int*)(h->d_blob + off_ent_combo);
+    h->bt.gcombo = h->d_blob + off_gcombo;
+    h->bt.ent_rc = (const int*)(h->d_blob + off_ent_rc);
+    h->bt.n_contrib = (int)n_contrib;
     // the wave-per-state kernels keep a state's arrays in 64 KiB of LDS at most
     const size_t lds_cap = 64 * 1024;
     h->wave_ok_rates = packable && pjk_host::wave_lds_bytes(m, false, 0) <= lds_cap;
@@ -602,11 +733,16 @@ int pyjac_hip_create(const pyjac_mech_tables* mt, int conp, char order, int jac_
     // the block kernels fit one workgroup's arrays into the CU's LDS
     h->block_ok_rates = packable && pjk_host::block_lds_bytes(m, false, h->bt.n_pairs) <= LDS_PER_CU;
     h->block_ok_jac = packable && pjk_host::block_lds_bytes(m, true, h->bt.n_pairs) <= LDS_PER_CU;
-    for (int k = 0; k < 2; ++k) {
-        h->lds[k] = pjk_host::block_lds(m, k == 1, h->bt.n_pairs);
-        pjk_host::BlockArgs a{m, h->bt, h->lds[k]};
-        hipError_t e2 = hipMalloc(&h->d_args[k], sizeof(a));
-        if (e2 == hipSuccess) e2 = hipMemcpy(h->d_args[k], &a, sizeof(a), hipMemcpyHostToDevice);
+    h->glob_ok_jac = packable && glob_tables &&
+                     pjk_host::block_lds_bytes(m, true, h->bt.n_pairs, false) <= LDS_PER_CU;
+    for (int k = 0; k < 3; ++k) {
+        pjk_host::BlockLds lay = pjk_host::block_lds(m, k != 0, h->bt.n_pairs, k != 2);
+        pjk_host::BlockArgs a{m, h->bt, lay};
+        pjk_host::BlockArgs** dst = k < 2 ? &h->d_args[k] : &h->d_args_g;
+        if (k < 2) h->lds[k] = lay;
+        else h->lds_g = lay;
+        hipError_t e2 = hipMalloc(dst, sizeof(a));
+        if (e2 == hipSuccess) e2 = hipMemcpy(*dst, &a, sizeof(a), hipMemcpyHostToDevice);
         if (e2 != hipSuccess) {
             pyjac_hip_destroy(h);
             return fail(PYJAC_ERR_HIP, "argument upload failed: %s", hipGetErrorString(e2));
@@ -620,6 +756,8 @@ int pyjac_hip_destroy(pyjac_handle* h) {
     if (!h) return PYJAC_OK;
     (void)hipFree(h->d_args[0]);
     (void)hipFree(h->d_args[1]);
+    (void)hipFree(h->d_args_g);
+    (void)hipFree(h->d_chunk);
     (void)hipFree(h->d_blob);
     (void)hipFree(h->d_ws);
     (void)hipFree(h->d_param);
@@ -860,9 +998,12 @@ int pyjac_hip_autotune(pyjac_handle* h, int kernel, size_t n, const double* d_pa
     bool wave = kernel == 0 ? h->wave_ok_rates : h->wave_ok_jac;
     if (block)
         for (int w : {1, 2, 4, 8}) cand.push_back(10 + w);
+    if (kernel == 1 && h->glob_ok_jac)
+        for (int w : {2, 4, 8}) cand.push_back(20 + w);
     if (wave) cand.push_back(2);
     cand.push_back(1);
     const int saved_variant = h->variant[kernel], saved_wps = h->wps[kernel];
+    const int saved_wps_g = h->wps_g;
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
@@ -893,9 +1034,14 @@ int pyjac_hip_autotune(pyjac_handle* h, int kernel, size_t n, const double* d_pa
     if (rc) {
         h->variant[kernel] = saved_variant;
         h->wps[kernel] = saved_wps;
+        h->wps_g = saved_wps_g;
         return rc;
     }
-    if (best > 10) {
+    if (best > 20) {
+        // automatic choice only when the dense accumulator does not fit
+        h->variant[kernel] = h->block_ok_jac ? best : 0;
+        h->wps_g = best - 20;
+    } else if (best > 10) {
         h->variant[kernel] = 0;
         h->wps[kernel] = best - 10;
     } else {
@@ -908,6 +1054,18 @@ int pyjac_hip_autotune(pyjac_handle* h, int kernel, size_t n, const double* d_pa
 
 int pyjac_hip_set_variant(pyjac_handle* h, int kernel, int variant) {
     if (!h || kernel < 0 || kernel > 3 || variant < 0) return fail(PYJAC_ERR_ARG, "bad variant");
+    if (kernel <= 1) {
+        const bool wave = kernel == 0 ? h->wave_ok_rates : h->wave_ok_jac;
+        const bool block = kernel == 0 ? h->block_ok_rates : h->block_ok_jac;
+        const int w = variant % 10;
+        const bool wok = w == 1 || w == 2 || w == 4 || w == 8;
+        bool ok = variant == 0 || variant == 1 || (variant == 2 && wave) ||
+                  (variant > 10 && variant < 20 && wok && block) ||
+                  (kernel == 1 && variant > 20 && variant < 30 && wok && h->glob_ok_jac);
+        if (!ok)
+            return fail(PYJAC_ERR_UNSUPPORTED, "variant %d cannot run this mechanism's %s", variant,
+                        kernel == 0 ? "species rates" : "Jacobian");
+    }
     h->variant[kernel] = variant;
     return PYJAC_OK;
 }

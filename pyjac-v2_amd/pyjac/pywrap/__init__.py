@@ -198,8 +198,8 @@ class _KernelBase(object):
     def _apply_tuned(self):
         choice = _tune_load().get(self._tune_key())
         if choice is not None:
-            check(self.lib.pyjac_hip_set_variant(self._h, self._variant_slot,
-                                                 int(choice['variant'])))
+            # a cached choice this build cannot run keeps the automatic one
+            self.lib.pyjac_hip_set_variant(self._h, self._variant_slot, int(choice['variant']))
 
     def _run(self, host_fn, dev_fn, param, phi, out):
         pp, dp = _ptr(param)

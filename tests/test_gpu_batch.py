@@ -36,6 +36,20 @@ def _setup(mechs, jac):
     return kernels, arrays
 
 
+def _same(o, r, jac):
+    """Rates: bit-identical.  Jacobians: the variant that accumulates in
+    device memory (the large mechanisms) adds with L2 atomics, whose order
+    varies from run to run, so each state agrees to 1e-13 (relative
+    Frobenius) instead."""
+    if not jac or torch.equal(o, r):
+        return torch.equal(o, r)
+    diff = o != r
+    if not bool(torch.isfinite(r[diff]).all()):
+        return False  # a non-finite entry that differs
+    d = (o[diff] - r[diff]).abs().max().item()
+    return d <= 1e-13 * r[torch.isfinite(r)].abs().max().item()
+
+
 def _eager(kernels, arrays):
     res = []
     for k, (p, f, _) in zip(kernels, arrays):
@@ -59,7 +73,7 @@ def test_batch_equals_single_calls(mechs, jac):
         b.run(graph=graph)
         torch.cuda.synchronize()
         for (_, _, o), r in zip(arrays, ref):
-            assert torch.equal(o, r)
+            assert _same(o, r, jac)
 
 
 def test_graph_replays_follow_new_inputs(mechs):
@@ -81,7 +95,7 @@ def test_graph_replays_follow_new_inputs(mechs):
         b.run(graph=True)
         torch.cuda.synchronize()
         for (_, _, o), r in zip(arrays, ref):
-            assert torch.equal(o, r), rep
+            assert _same(o, r, True), rep
 
 
 def test_autotune_keeps_results_right(mechs, tmp_path, monkeypatch):
@@ -93,7 +107,7 @@ def test_autotune_keeps_results_right(mechs, tmp_path, monkeypatch):
         k = K(n, 0, mech=mech, conp=False, order='F', jac_format=1)
         out = torch.empty(nv * n, dtype=torch.float64, device='cuda')
         v, ms = k.autotune(p, f, out)
-        assert v in (1, 2, 11, 12, 14, 18) and ms > 0
+        assert v in (1, 2, 11, 12, 14, 18, 22, 24, 28) and ms > 0
         assert (tmp_path / 'tune.json').exists()
         k2 = K(n, 0, mech=mech, conp=False, order='F', jac_format=1)  # picks the tuned choice
         out2 = torch.empty_like(out)

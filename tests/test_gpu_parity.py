@@ -91,8 +91,9 @@ LARGE = [('s150', True, 'C', 0), ('s150', False, 'F', 1), ('s500', True, 'F', 1)
 @pytest.mark.parametrize('name,conp,order,fmt', LARGE)
 def test_large_mechanisms(mechs, name, conp, order, fmt):
     """Seeded synthetic mechanisms of 150 and 500 species: species rates and
-    Jacobians (whose dense accumulator exceeds LDS, so the lane-per-state
-    kernels take them)."""
+    Jacobians (whose dense accumulator exceeds LDS, so the block kernels that
+    accumulate in device memory take them, through the chunked transpose in
+    F order), and the lane-per-state kernels on the same states."""
     mech = mechs(name)
     n = 96
     param, phi, phi_in = _states(mech, n, conp, order, seed=17)
@@ -108,5 +109,29 @@ def test_large_mechanisms(mechs, name, conp, order, fmt):
     jk(param, phi_in, jac)
     a = jac.reshape(n, -1) if order == 'C' else jac.reshape(-1, n).T
     b = jref.reshape(n, -1) if order == 'C' else jref.reshape(-1, n).T
+    fro = np.linalg.norm(a - b, axis=1) / np.linalg.norm(b, axis=1)
+    assert fro.max() <= JAC_FRO, fro.max()
+    jk.set_variant(1)
+    jac1 = np.zeros_like(jref)
+    jk(param, phi_in, jac1)
+    a1 = jac1.reshape(n, -1) if order == 'C' else jac1.reshape(-1, n).T
+    fro1 = np.linalg.norm(a1 - b, axis=1) / np.linalg.norm(b, axis=1)
+    assert fro1.max() <= JAC_FRO, fro1.max()
+
+
+def test_chunked_f_order(mechs, monkeypatch):
+    """F-order Jacobians of the device-memory variant go through chunks of
+    states and a transpose per chunk: 200 states in chunks of 64 (the last
+    one partial) match the checker."""
+    monkeypatch.setenv('PYJAC_HIP_CHUNK_STATES', '64')
+    mech = mechs('s150')
+    n = 200
+    param, phi, phi_in = _states(mech, n, False, 'F', seed=23)
+    jref = oracle.to_layout(mech, oracle.jacobian(mech, False, param, phi), 'F', 1)
+    jk = PyJacobianKernel(n, 0, mech=mech, conp=False, order='F', jac_format=1)
+    jac = np.zeros_like(jref)
+    jk(param, phi_in, jac)
+    a = jac.reshape(-1, n).T
+    b = jref.reshape(-1, n).T
     fro = np.linalg.norm(a - b, axis=1) / np.linalg.norm(b, axis=1)
     assert fro.max() <= JAC_FRO, fro.max()

@@ -58,20 +58,21 @@ struct BlockTables {
     int n_pairs, n_combo;
     // variant gathering in device memory (dense Jacobian larger than LDS)
     const int* acc_map;      // [NN * NN] dense index -> output entry, -1 if not stored
-    const int* rec_pos;      // [Nr][8][WAVE_MAXSP]: per record, the contribution slot of
-                             // (column slot 0-5 / T / extra variable, row slot), -1 for none
-    const int* ent_start;    // [jac_nnz + 1]: contribution slots of each output entry
-    const int* contrib_ent;  // [n_contrib]: output entry of each contribution slot
-    const int* ent_combo;    // [jac_nnz + 1]: third-body columns of each output entry
-    const void* gcombo;      // int4 [n_combo]: pair, output entry, efficiency (by entry)
+    const int* rec_pos;      // [Nr][8][WAVE_MAXSP]: per record, where the contribution of
+                             // (species column slot, row slot) goes: buffer slot >= 0,
+                             // LDS slot -2 - p, or -1 for none
+    const int* ent_start;    // [jac_nnz + 1]: buffer slots of each output entry (<= GC)
+    const int* ent_long;     // [jac_nnz]: LDS slot of an entry with more contributions, or -1
+    const void* gcombo;      // int4 [n_combo]: pair, target (as rec_pos), efficiency
     const int* ent_rc;       // [jac_nnz]: (row << 16) | column of each output entry
-    int n_contrib;
+    int n_contrib, n_long;
 };
 
 // LDS offsets (in doubles) of one workgroup's arrays
 struct BlockLds {
     int conc, nmol, b, db, wdot, energy, heat, dheat, U;
     int ci, nfac, dTf, dEf, Cp, mw, ksp, dkT, dkP, jac, total;
+    int aux;  // device-memory variant: columns T, V|P and the many-contribution entries
 };
 
 // everything the block kernels read besides the states: one copy in device
@@ -84,8 +85,11 @@ struct BlockArgs {
 };
 
 // dense: the Jacobian accumulator in LDS (false: the device-memory variant)
-BlockLds block_lds(const DevMech& m, bool jac, int n_pairs, bool dense = true);
-size_t block_lds_bytes(const DevMech& m, bool jac, int n_pairs, bool dense = true);
+BlockLds block_lds(const DevMech& m, bool jac, int n_pairs, bool dense = true, int n_long = 0);
+size_t block_lds_bytes(const DevMech& m, bool jac, int n_pairs, bool dense = true,
+                       int n_long = 0);
+// contributions an entry of the device-memory variant gathers from the buffer
+constexpr int GATHER_MAX = 4;
 
 // one workgroup of 64 * wps threads per state
 hipError_t launch_rates_block(const DevMech& m, const BlockArgs* args, const BlockLds& L, int wps,

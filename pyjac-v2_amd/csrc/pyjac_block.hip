@@ -659,14 +659,17 @@ __device__ __forceinline__ void scatter_wdot(Ctx& c, const Rec& x, double net, i
 }
 
 // Where a reaction's Jacobian contributions go: added into a dense NN x NN
-// array in LDS (mechanisms whose state fits the CU's LDS), or stored, one
-// slot each, into the state's contribution buffer in device memory (larger
-// mechanisms).  The host orders the buffer's slots by output entry, so a
-// later gather sums each entry's contiguous run: no atomics, and a fixed
-// summation order.
+// array in LDS (mechanisms whose state fits the CU's LDS), or, for larger
+// mechanisms, stored one slot each into the state's contribution buffer in
+// device memory, whose slots the host groups by output entry (at most GC per
+// entry, summed in slot order by a later gather).  Entries with more
+// contributions than that (diagonals of frequent species) get an LDS slot
+// that the contributions are added into instead.
 struct JacAcc {
     lds_double* lds;
     double* g;
+    lds_double *col0, *col1;  // device-memory variant: columns T and V|P in LDS,
+    lds_double* lng;          // and the entries with many contributions
 };
 
 // one contribution: to the dense LDS entry, or to the buffer slot that the
@@ -679,6 +682,7 @@ __device__ __forceinline__ void jput(const JacAcc& a, const int* __restrict__ po
     } else {
         const int e = pos[slot];
         if (e >= 0) a.g[e] = v;
+        else if (e < -1) lds_add(a.lng + (-2 - e), v);
     }
 }
 
@@ -819,8 +823,17 @@ __device__ __forceinline__ void reaction_jacobian(Ctx& c, const State& st, const
 #pragma unroll
     for (int k = 0; k < MAXSP; ++k) {
         if (k >= cm) break;
-        jput<G>(acc, ent, 6 * MAXSP + k, roff[k], nuk[k] * dT);
-        jput<G>(acc, ent, 7 * MAXSP + k, roff[k] + 1, nuk[k] * dE);
+        if constexpr (G) {
+            // columns T and V|P of the species rows stay in LDS
+            const int spec = x.spec(k);
+            const bool rowok = k < x.cnt && spec != ns && x.nu(k) != 0;
+            const int row = rowok ? spec : x.home;
+            lds_add(acc.col0 + row, nuk[k] * dT);
+            lds_add(acc.col1 + row, nuk[k] * dE);
+        } else {
+            jput<G>(acc, ent, 6 * MAXSP + k, roff[k], nuk[k] * dT);
+            jput<G>(acc, ent, 7 * MAXSP + k, roff[k] + 1, nuk[k] * dE);
+        }
     }
     if (gf & (GF_NS | GF_THD)) {
 #pragma unroll
@@ -941,7 +954,7 @@ __device__ __forceinline__ void jacobian_state(const BlockArgs* __restrict__ A, 
     Ctx c = make_ctx(A, smem, conp);
     const int Ns = c.Ns, NN = c.NN, ns = c.ns, Nr = c.m.Nr, nout = c.m.jac_nnz;
     lds_double* J = c.J();
-    const JacAcc acc{J, nullptr};
+    const JacAcc acc{J, nullptr, nullptr, nullptr};
     if (tid == 0) { c.conc(Ns) = 1.0; c.b(Ns) = 0.0; c.db(Ns) = 0.0; }
     for (int k = tid; k < Ns - 1; k += NT) c.mw(k) = c.m.mw_post[k];
     const double* ps = phi + s * L.sN;
@@ -1117,13 +1130,15 @@ __device__ __forceinline__ void jacobian_state(const BlockArgs* __restrict__ A, 
     }
 }
 
-// entries per gather tile of the device-memory variant (LDS doubles)
-constexpr int GTILE = 2048;
+// gather of the device-memory variant: entries per thread in flight, and
+// the contributions of each loaded together
+constexpr int GE = 8, GC = pjk_host::GATHER_MAX;
 
-// Agent-coherent load: the data was stored by other waves of the workgroup
-// in this launch, so the read must not be served by a stale L1 line.
-__device__ __forceinline__ double coh_load(const double* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+// Load of data stored by other waves of this workgroup (after their stores
+// completed and a barrier): workgroup scope, served by this CU's caches.
+// (Agent scope would bypass the XCD's L2 and read from memory.)
+__device__ __forceinline__ double wg_load(const double* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
 // Every memory operation of this wave completed before the barrier that
@@ -1150,19 +1165,14 @@ __device__ __forceinline__ void jacobian_global_state(const BlockArgs* __restric
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     Ctx c = make_ctx(A, smem, conp);
     const int Ns = c.Ns, NN = c.NN, ns = c.ns, Nr = c.m.Nr, nout = c.m.jac_nnz;
-    const JacAcc acc{nullptr, cb};
     lds_double* cs0 = c.sm + c.L.jac;  // column sums of the T row
     lds_double* cs1 = cs0 + NN;        // and of the extra-variable row
-    lds_double* col0 = cs1 + NN;       // columns 0 and 1 of the species rows
+    lds_double* col0 = c.sm + c.L.aux; // columns 0 and 1 of the species rows
     lds_double* col1 = col0 + NN;
+    lds_double* lng = col1 + NN;       // entries with many contributions
+    const JacAcc acc{nullptr, cb, col0, col1, lng};
     if (tid == 0) { c.conc(Ns) = 1.0; c.b(Ns) = 0.0; c.db(Ns) = 0.0; }
-    for (int k = tid; k < NN; k += NT) {
-        if (k < Ns - 1) c.mw(k) = c.m.mw_post[k];
-        cs0[k] = 0.0;
-        cs1[k] = 0.0;
-        col0[k] = 0.0;
-        col1[k] = 0.0;
-    }
+    for (int k = tid; k < Ns - 1; k += NT) c.mw(k) = c.m.mw_post[k];
     for (int p = tid; p < c.bt.n_pairs; p += NT) c.Cp(p) = 0.0;
     const double* ps = phi + s * L.sN;
     auto ph = [&](int i) { return ps[i * L.sI]; };
@@ -1171,6 +1181,10 @@ __device__ __forceinline__ void jacobian_global_state(const BlockArgs* __restric
     __syncthreads();
     bath_gas(c, st, lane);
     phase_pdep<true, WPS>(c, st, wave, lane);
+    __syncthreads();
+    // nmol is no longer read: its space takes columns T and V|P and the
+    // many-contribution entries
+    for (int i = tid; i < 2 * NN + c.bt.n_long; i += NT) col0[i] = 0.0;
     __syncthreads();
     // phase 3: reactions
     for (int pass = 0; pass * NT < Nr; ++pass) {
@@ -1183,6 +1197,20 @@ __device__ __forceinline__ void jacobian_global_state(const BlockArgs* __restric
             scatter_wdot(c, x, ropi * v.ci, gf);
             reaction_jacobian<true>(c, st, x, v, ropi, gf, acc,
                                     c.bt.rec_pos + (size_t)r * (8 * MAXSP));
+        }
+    }
+    __syncthreads();
+    // the reaction-only arrays are dead: their space takes the column sums
+    for (int k = tid; k < 2 * NN; k += NT) cs0[k] = 0.0;
+    // third-body columns (create_jacobian.py dci_*_dnj): each (pair, column)
+    // of the host's list is one more contribution of its entry
+    {
+        const int4* cmb = (const int4*)c.bt.gcombo;
+        for (int i = tid; i < c.bt.n_combo; i += NT) {
+            const int4 m4 = cmb[i];
+            const double v = c.Cp(m4.x) * __hiloint2double(m4.w, m4.z);
+            if (m4.y >= 0) cb[m4.y] = v;
+            else lds_add(lng + (-2 - m4.y), v);
         }
     }
     wait_memory();
@@ -1202,47 +1230,55 @@ __device__ __forceinline__ void jacobian_global_state(const BlockArgs* __restric
     const double extra = conp ? st.V : st.P;
     const double e_ns = c.energy(ns), h_ns = c.heat(ns);
     const double inv_vh = 1.0 / (st.V * heat_tot);
-    // phase 4: gather the stored species-row entries, GTILE entries at a
-    // time: the contributions of a tile's entries are contiguous, so the
-    // threads stride over them with coalesced loads and add each into its
-    // entry's LDS slot; then each entry of the tile is finished
+    // phase 4: gather the stored species-row entries.  An entry's buffer
+    // contributions (at most GC) are contiguous; a thread takes GE entries
+    // at a time with all their contributions in flight together and sums
+    // each in slot order; entries with more contributions are in LDS
     {
         const int* es = c.bt.ent_start;
-        const int* ce = c.bt.contrib_ent;
-        const int* ec = c.bt.ent_combo;
         const int* rc = c.bt.ent_rc;
-        const int4* cmb = (const int4*)c.bt.gcombo;
-        lds_double* tile = col1 + NN;
-        for (int t0 = 0; t0 < nout; t0 += GTILE) {
-            const int t1 = min(nout, t0 + GTILE);
-            for (int i = tid; i < t1 - t0; i += NT) tile[i] = 0.0;
-            __syncthreads();
-            const int q0 = es[t0], q1 = es[t1];
-            for (int q = q0 + tid; q < q1; q += NT) lds_add(tile + (ce[q] - t0), coh_load(cb + q));
-            __syncthreads();
-            for (int e = t0 + tid; e < t1; e += NT) {
-                const int t = rc[e];
-                const int row = t >> 16, col = t & 0xffff;
-                if (row < 2) continue;  // rows 0 and 1 come last
-                double v = tile[e - t0];
-                for (int q = ec[e], q1 = ec[e + 1]; q < q1; ++q) {
-                    const int4 m4 = cmb[q];
-                    v += c.Cp(m4.x) * __hiloint2double(m4.w, m4.z);
-                }
+        const int* el = c.bt.ent_long;
+        for (int eb = tid; eb < nout; eb += NT * GE) {
+            int trc[GE], a[GE], b[GE], lg[GE];
+#pragma unroll
+            for (int u = 0; u < GE; ++u) {
+                const int e = eb + u * NT;
+                const bool in = e < nout;
+                trc[u] = in ? rc[e] : 0;
+                a[u] = in ? es[e] : 0;
+                b[u] = in ? es[e + 1] : 0;
+                lg[u] = in ? el[e] : -1;
+            }
+            double x[GE][GC];
+#pragma unroll
+            for (int u = 0; u < GE; ++u)
+#pragma unroll
+                for (int i = 0; i < GC; ++i)
+                    x[u][i] = a[u] + i < b[u] ? wg_load(cb + a[u] + i) : 0.0;
+#pragma unroll
+            for (int u = 0; u < GE; ++u) {
+                const int e = eb + u * NT;
+                const int row = trc[u] >> 16, col = trc[u] & 0xffff;
+                if (row < 2) continue;  // rows 0 and 1 come last (and padding)
                 const int k = row - 2;
+                double v;
                 if (col >= 2) {
+                    if (lg[u] >= 0) {
+                        v = lng[lg[u]];
+                    } else {
+                        v = x[u][0];
+#pragma unroll
+                        for (int i = 1; i < GC; ++i) v += x[u][i];
+                    }
                     const double mw = c.mw(k);
                     lds_add(cs0 + col - 2, (c.energy(k) + -1.0 * e_ns * mw) * v);
                     lds_add(cs1 + col - 2, (1.0 + -1.0 * mw) * v);
                     v += c.U(k);
-                } else if (col == 0) {
-                    col0[k] = v;
                 } else {
-                    col1[k] = v;
+                    v = col == 0 ? col0[k] : col1[k];
                 }
                 out[e] = v;
             }
-            __syncthreads();
         }
     }
     // the bath-gas sums join every species column of their row
@@ -1435,15 +1471,46 @@ __global__ void __launch_bounds__(64 * WPS) jacobian_ragged_kernel(RaggedArgs R)
 
 namespace pjk_host {
 
-BlockLds block_lds(const DevMech& m, bool jac, int bt_pairs, bool dense) {
+BlockLds block_lds(const DevMech& m, bool jac, int bt_pairs, bool dense, int bt_long) {
     BlockLds L{};
     int o = 0;
-    const int Ns = m.Ns;
+    const int Ns = m.Ns, NN = m.NN;
     auto take = [&](int cnt) {
         int at = o;
         o += cnt > 0 ? cnt : 1;
         return at;
     };
+    if (jac && !dense) {
+        // device-memory variant: arrays whose lifetimes do not overlap share
+        // space.  nmol (until the bath-gas sum) lies under columns T and V|P
+        // and the many-contribution entries (from the reaction pass on); the
+        // column sums of rows 0 and 1 (gather) lie under the arrays only the
+        // reaction pass reads.
+        L.conc = take(Ns + 1);
+        L.wdot = take(Ns + 1);
+        L.energy = take(Ns);
+        L.heat = take(Ns);
+        L.dheat = take(Ns);
+        L.U = take(Ns);
+        L.mw = take(Ns - 1);
+        L.Cp = take(bt_pairs);
+        L.aux = take(2 * NN + bt_long);
+        L.nmol = L.aux;
+        const int r0 = o;
+        L.b = take(Ns + 1);
+        L.db = take(Ns + 1);
+        L.ci = take(m.n_thd);
+        L.ksp = take(m.n_plog + m.n_cheb);
+        L.nfac = take(m.n_thd);
+        L.dTf = take(m.n_thd);
+        L.dEf = take(m.n_thd);
+        L.dkT = take(m.n_plog + m.n_cheb);
+        L.dkP = take(m.n_plog + m.n_cheb);
+        if (o - r0 >= 2 * NN) L.jac = r0;
+        else L.jac = take(2 * NN);
+        L.total = o;
+        return L;
+    }
     L.conc = take(Ns + 1);
     L.nmol = take(Ns);
     L.b = take(Ns + 1);
@@ -1464,17 +1531,14 @@ BlockLds block_lds(const DevMech& m, bool jac, int bt_pairs, bool dense) {
         L.dkT = take(m.n_plog + m.n_cheb);
         L.dkP = take(m.n_plog + m.n_cheb);
         o = (o + 1) & ~1;  // 16-byte alignment of the accumulator
-        // dense accumulator, or (variant gathering in device memory) the
-        // column sums of rows 0 and 1, columns 0 and 1 of the species rows
-        // and the gather tile
-        L.jac = take(dense ? m.NN * m.NN : 4 * m.NN + pjb::GTILE);
+        L.jac = take(m.NN * m.NN);
     }
     L.total = o;
     return L;
 }
 
-size_t block_lds_bytes(const DevMech& m, bool jac, int n_pairs, bool dense) {
-    return (size_t)block_lds(m, jac, n_pairs, dense).total * sizeof(double);
+size_t block_lds_bytes(const DevMech& m, bool jac, int n_pairs, bool dense, int n_long) {
+    return (size_t)block_lds(m, jac, n_pairs, dense, n_long).total * sizeof(double);
 }
 
 template <class K>

@@ -540,66 +540,63 @@ int pyjac_hip_create(const pyjac_mech_tables* mt, int conp, char order, int jac_
         out_pos[2 * e + 1] = (row >= 2 && col >= 2) ? row - 2 : -1;
     }
     // tables of the variant gathering in device memory: dense index ->
-    // output entry; every reaction's contribution slots, grouped by output
-    // entry (ent_start) in record order, and each record's slot table; the
-    // third-body columns sorted by output entry (ent_combo); each entry's
-    // row and column
+    // output entry; each entry's contribution slots (ent_start, in record
+    // order, then its third-body columns) or LDS slot (ent_long); each
+    // record's and each third-body column's target; each entry's row and
+    // column
     std::vector<int> acc_map((size_t)NN * NN, -1);
     for (size_t e = 0; e < jac_pos.size(); ++e) acc_map[jac_pos[e]] = (int)e;
     const size_t nout_ = jac_pos.size();
     bool glob_tables = NN <= 0x7fff;
     // rows 0 and 1 are stored in full (the kernel writes them by entry)
     for (int j = 0; j < NN; ++j) glob_tables = glob_tables && acc_map[j] >= 0 && acc_map[NN + j] >= 0;
+    // contributions per entry: the reactions' species-column slots (columns
+    // T and V|P stay in LDS) and the third-body columns; entries with more
+    // than GATHER_MAX get an LDS slot instead of buffer slots
     const int MS = pjk_host::WAVE_MAXSP;
     std::vector<int> rec_pos((size_t)Nr * 8 * MS, -1);
-    std::vector<long> ent_count(nout_ + 1, 0);
-    for (int pass = 0; pass < 2 && packable; ++pass) {
-        // pass 0 counts the contributions of each entry, pass 1 places them
-        std::vector<long> next;
-        if (pass == 1) next.assign(ent_count.begin(), ent_count.end() - 1);
+    auto each_slot = [&](auto&& fn) {
         for (int q = 0; q < Nr; ++q) {
             const int r = perm[q], e0 = mt->rxn_off[r], cnt = mt->rxn_off[r + 1] - e0;
-            for (int k = 0; k < cnt; ++k) {
+            for (int k = 0; k < cnt && k < MS; ++k) {
                 const int sk = mt->rxn_spec[e0 + k];
                 if (sk == Ns - 1 || mt->nu_prod[e0 + k] == mt->nu_reac[e0 + k]) continue;
                 const size_t rowp = (size_t)(sk + 2) * NN;
-                for (int j = 0; j < 8; ++j) {
-                    int dense;
-                    if (j < MS) {
-                        if (j >= cnt || mt->rxn_spec[e0 + j] == Ns - 1) continue;
-                        dense = (int)rowp + mt->rxn_spec[e0 + j] + 2;
-                    } else {
-                        dense = (int)rowp + (j - MS);
-                    }
+                for (int j = 0; j < cnt && j < MS; ++j) {
+                    if (mt->rxn_spec[e0 + j] == Ns - 1) continue;
                     // an unstored pair is a structural zero (e.g. the product
                     // columns of an irreversible reaction): nothing to store
-                    const int e = acc_map[dense];
-                    if (e < 0) continue;
-                    if (pass == 0) ++ent_count[e + 1];
-                    else rec_pos[((size_t)q * 8 + j) * MS + k] = (int)next[e]++;
+                    const int e = acc_map[rowp + mt->rxn_spec[e0 + j] + 2];
+                    if (e >= 0) fn(e, ((size_t)q * 8 + j) * MS + k);
                 }
             }
         }
-        if (pass == 0)
-            for (size_t e = 0; e < nout_; ++e) ent_count[e + 1] += ent_count[e];
-    }
-    const long n_contrib = ent_count[nout_];
-    glob_tables = glob_tables && n_contrib < (1L << 31);
-    std::vector<int> ent_start(ent_count.begin(), ent_count.end());
-    std::vector<int> contrib_ent(std::max<long>(n_contrib, 1), 0);
-    for (size_t e = 0; e < nout_; ++e)
-        for (long q = ent_count[e]; q < ent_count[e + 1]; ++q) contrib_ent[q] = (int)e;
+    };
     std::vector<Combo> gcombos(combos);
     for (auto& cb : gcombos) {
         cb.dst = acc_map[cb.dst];
         glob_tables = glob_tables && cb.dst >= 0;
     }
-    std::stable_sort(gcombos.begin(), gcombos.end(),
-                     [](const Combo& a, const Combo& b) { return a.dst < b.dst; });
-    std::vector<int> ent_combo(nout_ + 1, 0);
-    for (const auto& cb : gcombos)
-        if (cb.dst >= 0) ++ent_combo[cb.dst + 1];
-    for (size_t e = 0; e < nout_; ++e) ent_combo[e + 1] += ent_combo[e];
+    std::vector<long> ent_n(nout_, 0);
+    if (packable && glob_tables) {
+        each_slot([&](int e, size_t) { ++ent_n[e]; });
+        for (const auto& cb : gcombos) ++ent_n[cb.dst];
+    }
+    std::vector<int> ent_long(nout_, -1), ent_start(nout_ + 1, 0);
+    int n_long = 0;
+    for (size_t e = 0; e < nout_; ++e) {
+        if (ent_n[e] > pjk_host::GATHER_MAX) ent_long[e] = n_long++;
+        ent_start[e + 1] = ent_start[e] + (ent_long[e] >= 0 ? 0 : (int)ent_n[e]);
+    }
+    const long n_contrib = ent_start[nout_];
+    glob_tables = glob_tables && n_contrib < (1L << 31);
+    {
+        std::vector<int> next(ent_start.begin(), ent_start.end() - 1);
+        auto target = [&](int e) { return ent_long[e] >= 0 ? -2 - ent_long[e] : next[e]++; };
+        if (packable && glob_tables) each_slot([&](int e, size_t at) { rec_pos[at] = target(e); });
+        for (auto& cb : gcombos)
+            if (cb.dst >= 0) cb.dst = target(cb.dst);
+    }
     std::vector<int> ent_rc(nout_);
     for (size_t e = 0; e < nout_; ++e) ent_rc[e] = ((jac_pos[e] / NN) << 16) | (jac_pos[e] % NN);
 
@@ -690,8 +687,7 @@ int pyjac_hip_create(const pyjac_mech_tables* mt, int conp, char order, int jac_
     size_t off_acc_map = P.add(acc_map.data(), acc_map.size());
     size_t off_rec_pos = P.add(rec_pos.data(), rec_pos.size());
     size_t off_ent_start = P.add(ent_start.data(), ent_start.size());
-    size_t off_contrib_ent = P.add(contrib_ent.data(), contrib_ent.size());
-    size_t off_ent_combo = P.add(ent_combo.data(), ent_combo.size());
+    size_t off_ent_long = P.add(ent_long.data(), ent_long.size());
     size_t off_gcombo = P.add(gcombos.data(), gcombos.size());
     size_t off_ent_rc = P.add(ent_rc.data(), ent_rc.size());
     hipError_t e = hipMalloc(&h->d_blob, P.bytes.size());
@@ -721,11 +717,11 @@ int pyjac_hip_create(const pyjac_mech_tables* mt, int conp, char order, int jac_
     h->bt.acc_map = (const int*)(h->d_blob + off_acc_map);
     h->bt.rec_pos = (const int*)(h->d_blob + off_rec_pos);
     h->bt.ent_start = (const int*)(h->d_blob + off_ent_start);
-    h->bt.contrib_ent = (const int*)(h->d_blob + off_contrib_ent);
-    h->bt.ent_combo = (const int*)(h->d_blob + off_ent_combo);
+    h->bt.ent_long = (const int*)(h->d_blob + off_ent_long);
     h->bt.gcombo = h->d_blob + off_gcombo;
     h->bt.ent_rc = (const int*)(h->d_blob + off_ent_rc);
     h->bt.n_contrib = (int)n_contrib;
+    h->bt.n_long = n_long;
     // the wave-per-state kernels keep a state's arrays in 64 KiB of LDS at most
     const size_t lds_cap = 64 * 1024;
     h->wave_ok_rates = packable && pjk_host::wave_lds_bytes(m, false, 0) <= lds_cap;
@@ -734,9 +730,17 @@ int pyjac_hip_create(const pyjac_mech_tables* mt, int conp, char order, int jac_
     h->block_ok_rates = packable && pjk_host::block_lds_bytes(m, false, h->bt.n_pairs) <= LDS_PER_CU;
     h->block_ok_jac = packable && pjk_host::block_lds_bytes(m, true, h->bt.n_pairs) <= LDS_PER_CU;
     h->glob_ok_jac = packable && glob_tables &&
-                     pjk_host::block_lds_bytes(m, true, h->bt.n_pairs, false) <= LDS_PER_CU;
+                     pjk_host::block_lds_bytes(m, true, h->bt.n_pairs, false, n_long) <= LDS_PER_CU;
+    if (getenv("PYJAC_HIP_VERBOSE"))
+        fprintf(stderr,
+                "pyjac_hip: Ns %d Nr %d nnz %d | LDS bytes: rates %zu, Jacobian %zu (dense), "
+                "%zu (device-memory variant, %ld contributions) | block %d/%d, device-memory %d\n",
+                Ns, Nr, (int)jac_pos.size(), pjk_host::block_lds_bytes(m, false, h->bt.n_pairs),
+                pjk_host::block_lds_bytes(m, true, h->bt.n_pairs),
+                pjk_host::block_lds_bytes(m, true, h->bt.n_pairs, false, n_long), n_contrib,
+                (int)h->block_ok_rates, (int)h->block_ok_jac, (int)h->glob_ok_jac);
     for (int k = 0; k < 3; ++k) {
-        pjk_host::BlockLds lay = pjk_host::block_lds(m, k != 0, h->bt.n_pairs, k != 2);
+        pjk_host::BlockLds lay = pjk_host::block_lds(m, k != 0, h->bt.n_pairs, k != 2, n_long);
         pjk_host::BlockArgs a{m, h->bt, lay};
         pjk_host::BlockArgs** dst = k < 2 ? &h->d_args[k] : &h->d_args_g;
         if (k < 2) h->lds[k] = lay;

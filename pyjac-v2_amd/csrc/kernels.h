@@ -96,9 +96,12 @@ hipError_t launch_rates_block(const DevMech& m, const BlockArgs* args, const Blo
                               int conp, char order, long n, const double* param,
                               const double* phi, double* dphi, hipStream_t stream);
 
+// F order with chunk_buf: C-order chunks of chunk_states states (a multiple
+// of 64) and a tiled transpose per chunk; otherwise written directly
 hipError_t launch_jacobian_block(const DevMech& m, const BlockArgs* args, const BlockLds& L,
                                  int wps, int conp, char order, long n, const double* param,
-                                 const double* phi, double* jac, hipStream_t stream);
+                                 const double* phi, double* jac, double* chunk_buf,
+                                 long chunk_states, hipStream_t stream);
 
 // one workgroup of 64 * wps threads per state, Jacobian gathered in device
 // memory, chunk_states states (a multiple of 64) per launch: chunk_buf holds

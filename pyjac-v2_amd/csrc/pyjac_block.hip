@@ -1403,7 +1403,39 @@ jacobian_global_kernel(const BlockArgs* __restrict__ A, int conp, long s0, long 
 
 // chunk buffer [count][nout] (entries of a state contiguous) -> F-order
 // output [nout][n], states s0 .. s0 + count - 1; 64 x 64 tiles through LDS so
-// that reads and writes are both whole lines
+// that reads and writes are both whole lines.  With nout, n and s0 even every
+// access is a 16-byte pair (entries_to_f2_kernel), otherwise single doubles.
+__global__ void __launch_bounds__(256)
+entries_to_f2_kernel(const double* __restrict__ src, long count, int nout,
+                     double* __restrict__ dst, long n, long s0) {
+    typedef double v2d __attribute__((ext_vector_type(2)));
+    __shared__ double tile[64][65];
+    const long t0 = (long)blockIdx.x * 64;
+    const int e0 = blockIdx.y * 64;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 8 rows per pass
+    for (int r = ty; r < 64; r += 8) {
+        const long st = t0 + r;
+        const int e = e0 + 2 * tx;
+        if (st < count && e < nout) {
+            const v2d v = *(const v2d*)(src + st * nout + e);  // nout even: e + 1 < nout
+            tile[r][2 * tx] = v.x;
+            tile[r][2 * tx + 1] = v.y;
+        }
+    }
+    __syncthreads();
+    for (int r = ty; r < 64; r += 8) {
+        const int e = e0 + r;
+        const long st = t0 + 2 * tx;
+        if (e < nout && st < count) {
+            if (st + 1 < count) {
+                *(v2d*)(dst + (size_t)e * n + s0 + st) = v2d{tile[2 * tx][r], tile[2 * tx + 1][r]};
+            } else {
+                dst[(size_t)e * n + s0 + st] = tile[2 * tx][r];
+            }
+        }
+    }
+}
+
 __global__ void __launch_bounds__(256)
 entries_to_f_kernel(const double* __restrict__ src, long count, int nout,
                     double* __restrict__ dst, long n, long s0) {
@@ -1574,9 +1606,68 @@ hipError_t launch_rates_block(const DevMech& m, const BlockArgs* args, const Blo
     return hipGetLastError();
 }
 
+// chunk of cnt states' entries (C order) -> F-order output columns s0 ..
+static hipError_t launch_entries_to_f(const double* chunk, long cnt, int nout, double* jac, long n,
+                                      long s0, hipStream_t stream) {
+    dim3 grid((unsigned)((cnt + 63) / 64), (unsigned)((nout + 63) / 64));
+    if (nout % 2 == 0 && n % 2 == 0 && s0 % 2 == 0 && (uintptr_t)chunk % 16 == 0 &&
+        (uintptr_t)jac % 16 == 0)
+        hipLaunchKernelGGL(pjb::entries_to_f2_kernel, grid, dim3(256), 0, stream, chunk, cnt,
+                           nout, jac, n, s0);
+    else
+        hipLaunchKernelGGL(pjb::entries_to_f_kernel, grid, dim3(256), 0, stream, chunk, cnt, nout,
+                           jac, n, s0);
+    return hipGetLastError();
+}
+
+// states s0 .. s0 + cnt - 1 of F-order inputs with n states, entries in C
+// order into out (cnt * nout)
+static hipError_t launch_jacobian_block_chunk(const DevMech& m, const BlockArgs* args,
+                                              const BlockLds& BL, int wps, int conp, long n,
+                                              long s0, long cnt, const double* param,
+                                              const double* phi, double* out,
+                                              hipStream_t stream) {
+    pjb::Layout L{1, (size_t)n};
+    size_t lds = (size_t)BL.total * sizeof(double);
+    const long grid = 8 * ((cnt + 7) / 8);
+#define PJB_JACC(W)                                                                       \
+    case W: {                                                                             \
+        hipError_t e = set_lds(pjb::jacobian_block_kernel<W>, lds);                      \
+        if (e != hipSuccess) return e;                                                    \
+        hipLaunchKernelGGL(pjb::jacobian_block_kernel<W>, dim3(grid), dim3(64 * W), lds,  \
+                           stream, args, conp, cnt, param + s0, phi + s0, out, L, 1);     \
+        break;                                                                            \
+    }
+    switch (wps) {
+        PJB_JACC(1)
+        PJB_JACC(2)
+        PJB_JACC(4)
+        PJB_JACC(8)
+        default: return hipErrorInvalidValue;
+    }
+#undef PJB_JACC
+    return hipGetLastError();
+}
+
 hipError_t launch_jacobian_block(const DevMech& m, const BlockArgs* args, const BlockLds& BL,
                                  int wps, int conp, char order, long n, const double* param,
-                                 const double* phi, double* jac, hipStream_t stream) {
+                                 const double* phi, double* jac, double* chunk_buf,
+                                 long chunk_states, hipStream_t stream) {
+    if (order == 'F' && chunk_buf) {
+        // F order through C-order chunks: each state's entries are written as
+        // one contiguous run, then a tiled transpose writes whole lines
+        if (chunk_states < 64 || chunk_states % 64) return hipErrorInvalidValue;
+        const int nout = m.jac_nnz;
+        for (long s0 = 0; s0 < n; s0 += chunk_states) {
+            const long cnt = std::min(chunk_states, n - s0);
+            hipError_t e = launch_jacobian_block_chunk(m, args, BL, wps, conp, n, s0, cnt, param,
+                                                       phi, chunk_buf, stream);
+            if (e != hipSuccess) return e;
+            e = launch_entries_to_f(chunk_buf, cnt, nout, jac, n, s0, stream);
+            if (e != hipSuccess) return e;
+        }
+        return hipSuccess;
+    }
     pjb::Layout L = order == 'F' ? pjb::Layout{1, (size_t)n} : pjb::Layout{(size_t)m.NN, 1};
     size_t lds = (size_t)BL.total * sizeof(double);
     const long grid = 8 * ((n + 7) / 8);
@@ -1631,12 +1722,8 @@ hipError_t launch_jacobian_global(const DevMech& m, const BlockArgs* args, const
             default: return hipErrorInvalidValue;
         }
 #undef PJB_GJ
-        if (order == 'F') {
-            dim3 grid((unsigned)((cnt + 63) / 64), (unsigned)((nout + 63) / 64));
-            hipLaunchKernelGGL(pjb::entries_to_f_kernel, grid, dim3(256), 0, stream,
-                               (const double*)ebuf, cnt, nout, jac, n, s0);
-        }
         hipError_t e = hipGetLastError();
+        if (e == hipSuccess && order == 'F') e = launch_entries_to_f(ebuf, cnt, nout, jac, n, s0, stream);
         if (e != hipSuccess) return e;
     }
     return hipSuccess;

@@ -75,6 +75,9 @@ struct pyjac_handle {
     int wps_g = 0;
     double* d_chunk = nullptr;  // F order: one chunk of states' entries
     long chunk_states = 0;
+    // LDS-kernel Jacobian in F order: C-order chunk before the transpose
+    double* d_fchunk = nullptr;
+    long fchunk_states = 0;
     // block kernels: device copies of their arguments, LDS layouts and the
     // tuned number of waves per state (0: default)
     pjk_host::BlockArgs* d_args[2] = {nullptr, nullptr};
@@ -149,7 +152,9 @@ int block_wps(const pyjac_handle* h, int kernel) {
     int v = h->variant[kernel];
     if (v > 10) return v - 10;
     if (h->wps[kernel]) return h->wps[kernel];
-    return 1;
+    // measured on MI355X (GRI, 1M states): rates fastest with one wave per
+    // state, the Jacobian with two
+    return kernel == 1 ? 2 : 1;
 }
 
 bool use_block(const pyjac_handle* h, int kernel) {
@@ -175,6 +180,31 @@ int glob_wps(const pyjac_handle* h) {
     return Nr >= 2000 ? 8 : (Nr >= 600 ? 4 : 2);
 }
 
+// F-order output of the LDS block kernel through a C-order chunk and a
+// transpose (PYJAC_HIP_FCHUNK states per chunk, 0 for direct F writes)
+long fchunk_want(const pyjac_handle* h, long n) {
+    if (h->order != 'F') return 0;
+    // up to 65536 states and 1 GiB (measured on MI355X, GRI: 47.2 ms per
+    // 1M states against 50.5 ms with direct F-order writes)
+    const size_t per = (size_t)h->m.jac_nnz * sizeof(double);
+    long want = std::min(65536L, (long)(((size_t)1 << 30) / per) / 64 * 64);
+    const char* env = getenv("PYJAC_HIP_FCHUNK");
+    if (env) want = atol(env) / 64 * 64;
+    if (want <= 0) return 0;
+    return std::max(64L, std::min(want, (n + 63) / 64 * 64));
+}
+
+int ensure_fchunk(pyjac_handle* h, long n) {
+    const long want = fchunk_want(h, n);
+    if (!want || (h->d_fchunk && h->fchunk_states >= want)) return PYJAC_OK;
+    (void)hipFree(h->d_fchunk);
+    h->d_fchunk = nullptr;
+    h->fchunk_states = 0;
+    HIP_TRY(hipMalloc(&h->d_fchunk, (size_t)want * h->m.jac_nnz * sizeof(double)));
+    h->fchunk_states = want;
+    return PYJAC_OK;
+}
+
 // chunk buffer of the device-memory variant (about 2 GiB): per state its
 // contributions and, in F order, its entries
 int ensure_chunk(pyjac_handle* h, long n) {
@@ -193,10 +223,15 @@ int ensure_chunk(pyjac_handle* h, long n) {
     return PYJAC_OK;
 }
 
-// workspace of the lane-per-state kernels for n states, allocated ahead of
-// a graph capture (no allocation may happen while a stream is captured)
+// device buffers a call on n states needs (lane-per-state workspace, chunk
+// buffers), allocated ahead of a graph capture (no allocation may happen
+// while a stream is captured)
 int prepare_lane(pyjac_handle* h, int kernel, long n) {
     if (kernel == 1 && use_glob(h)) return ensure_chunk(h, n);
+    if (kernel == 1 && use_block(h, 1)) return ensure_fchunk(h, n);
+    if (use_block(h, kernel)) return PYJAC_OK;
+    if (h->variant[kernel] == 2 && (kernel == 0 ? h->wave_ok_rates : h->wave_ok_jac))
+        return PYJAC_OK;
     size_t per = pjk_host::ws_doubles_per_state(h->m, kernel == 1);
     int grid = pick_grid(h, per, h->variant[kernel], n);
     return ensure_ws(h, per * (size_t)grid * h->block * sizeof(double));
@@ -229,8 +264,13 @@ int run_jac(pyjac_handle* h, long n, const double* param, const double* phi, dou
             hipStream_t s) {
     if (n <= 0) return PYJAC_OK;
     if (use_block(h, 1)) {
+        int rc = ensure_fchunk(h, n);
+        if (rc) return rc;
+        const bool chunked = fchunk_want(h, n) > 0;
         HIP_TRY(pjk_host::launch_jacobian_block(h->m, h->d_args[1], h->lds[1], block_wps(h, 1),
-                                                h->conp, h->order, n, param, phi, jac, s));
+                                                h->conp, h->order, n, param, phi, jac,
+                                                chunked ? h->d_fchunk : nullptr,
+                                                chunked ? h->fchunk_states : 0, s));
         return PYJAC_OK;
     }
     if (use_glob(h)) {
@@ -762,6 +802,7 @@ int pyjac_hip_destroy(pyjac_handle* h) {
     (void)hipFree(h->d_args[1]);
     (void)hipFree(h->d_args_g);
     (void)hipFree(h->d_chunk);
+    (void)hipFree(h->d_fchunk);
     (void)hipFree(h->d_blob);
     (void)hipFree(h->d_ws);
     (void)hipFree(h->d_param);
@@ -950,13 +991,9 @@ int pyjac_hip_batch_run(pyjac_batch* b, int kernel, int use_graph, void* stream)
         // allocations and kernel attributes first: nothing may allocate
         // while the stream is captured
         for (size_t i = 0; i < b->h.size(); ++i) {
-            pyjac_handle* h = b->h[i];
-            const bool wave = h->variant[kernel] == 2 &&
-                              (kernel == 0 ? h->wave_ok_rates : h->wave_ok_jac);
-            if (b->mem[i].n && !use_block(h, kernel) && !wave) {
-                int rc = prepare_lane(h, kernel, (long)b->mem[i].n);
-                if (rc) return rc;
-            }
+            if (!b->mem[i].n) continue;
+            int rc = prepare_lane(b->h[i], kernel, (long)b->mem[i].n);
+            if (rc) return rc;
         }
         int rc = batch_launches(b, kernel, b->cap);  // eager run: sets kernel attributes
         if (rc) return rc;

@@ -192,7 +192,8 @@ def main():
         'roofline': {'bound': 'hbm', 'achieved': achieved, 'peak': HBM_PEAK_GBS,
                      'unit': 'GB/s', 'frac': achieved / HBM_PEAK_GBS,
                      'traffic': measured_traffic(n),
-                     'kernel': 'jacobian_block_kernel',
+                     'kernel': 'Jacobian path: jacobian_block_kernel<2> (C-order chunks of '
+                               '65536 states) + entries_to_f2_kernel (transpose to F order)',
                      'algorithmic_bytes_per_state': bytes_per_state},
     }
     if not args.no_surrogate:

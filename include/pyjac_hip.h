@@ -56,6 +56,12 @@ extern "C" {
 #define PYJAC_JAC_SPARSE 1
 #define PYJAC_JAC_EXACT 0
 #define PYJAC_JAC_APPROXIMATE 1
+#define PYJAC_JAC_FINITE_DIFFERENCE 2
+
+/* finite-difference modes (FiniteDifferenceMode, core/enum_types.py) */
+#define PYJAC_FD_FORWARD 0
+#define PYJAC_FD_CENTRAL 1
+#define PYJAC_FD_BACKWARD 2
 
 /*
  * Mechanism tables.  Same content as the dict built by the reference's
@@ -146,7 +152,8 @@ const char *pyjac_hip_last_error(void);
  *   conp        1 constant pressure, 0 constant volume (utils.py:972)
  *   order       'C' or 'F' (utils.py:944)
  *   jac_format  PYJAC_JAC_FULL or PYJAC_JAC_SPARSE (utils.py:1014)
- *   jac_type    PYJAC_JAC_EXACT or PYJAC_JAC_APPROXIMATE (utils.py:999)
+ *   jac_type    PYJAC_JAC_EXACT, PYJAC_JAC_APPROXIMATE (unsupported) or
+ *               PYJAC_JAC_FINITE_DIFFERENCE (utils.py:999)
  */
 int pyjac_hip_create(const pyjac_mech_tables *mech, int conp, char order,
                      int jac_format, int jac_type, pyjac_handle **out);
@@ -183,6 +190,15 @@ int pyjac_hip_species_rates_dev(pyjac_handle *h, size_t n_states,
 int pyjac_hip_jacobian_dev(pyjac_handle *h, size_t n_states,
                            const double *d_param, const double *d_phi,
                            double *d_jac, void *stream);
+
+/* Options of a PYJAC_JAC_FINITE_DIFFERENCE handle (finite_difference_jacobian,
+   core/create_jacobian.py:4523-4526): mode PYJAC_FD_FORWARD / _BACKWARD
+   (order 1-6) or PYJAC_FD_CENTRAL (order 2, 4, 6, 8), and the relative and
+   absolute tolerances the steps come from.  Defaults: forward, order 1,
+   rtol 1e-8, atol 1e-15.  The Jacobian entry points of such a handle then
+   return the finite-difference Jacobian, evaluated with the species-rate
+   kernels on C-order chunks of perturbed states. */
+int pyjac_hip_set_fd_options(pyjac_handle *h, int mode, int order, double rtol, double atol);
 
 /* Introspection (Kernel::numSpecies etc., kernel.hpp.in:180-268). */
 int pyjac_hip_num_species(const pyjac_handle *h);

@@ -39,6 +39,8 @@ def lib():
         L.oracle_chem_utils.argtypes = [P, ctypes.c_int, ctypes.c_long, P, P, P, P]
         L.oracle_jacobian.argtypes = [P, ctypes.c_int, ctypes.c_long, P, P, P, ctypes.c_int]
         L.oracle_jacobian_ad.argtypes = [P, ctypes.c_int, ctypes.c_long, P, P, P]
+        L.oracle_fd_jacobian.argtypes = [P, ctypes.c_int, ctypes.c_long, P, P, P, ctypes.c_int,
+                                         ctypes.c_int, ctypes.c_double, ctypes.c_double]
         _lib = L
     return _lib
 
@@ -103,6 +105,19 @@ def jacobian_ad(mech, conp, param, phi):
     out = np.empty((n, mech.NN, mech.NN))
     st = lib().oracle_jacobian_ad(ctypes.cast(mech.pointer(), ctypes.c_void_p), int(conp), n,
                                   _p(param), _p(phi), _p(out))
+    assert st == 0, st
+    return out
+
+
+def fd_jacobian(mech, conp, param, phi, mode=0, order=1, rtol=1e-8, atol=1e-15):
+    """Dense finite-difference Jacobian ``[n, NN, NN]`` (mode 0 forward, 1
+    central, 2 backward)."""
+    param, phi = _prep(mech, param, phi)
+    n = phi.shape[0]
+    out = np.empty((n, mech.NN, mech.NN))
+    st = lib().oracle_fd_jacobian(ctypes.cast(mech.pointer(), ctypes.c_void_p), int(conp), n,
+                                  _p(param), _p(phi), _p(out), int(mode), int(order),
+                                  float(rtol), float(atol))
     assert st == 0, st
     return out
 

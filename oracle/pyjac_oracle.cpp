@@ -17,6 +17,7 @@
 // automatic-differentiation Jacobian (the reference's tests use Adept for
 // this, tests/test_jacobian.py:132-265).
 #include <algorithm>
+#include <cfloat>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -1307,6 +1308,88 @@ int oracle_jacobian_ad(const pyjac_mech_tables* m, int conp, long n,
             for (int row = 0; row < NN; ++row)
                 for (int q = 0; q < CH && c0 + q < NN; ++q)
                     jac[s * (size_t)NN * NN + row * NN + c0 + q] = y[row].d[q];
+        }
+    }
+    return PYJAC_OK;
+}
+
+// finite-difference Jacobian (create_jacobian.py:4523-4896), dense
+// jac[n][NN][NN]; mode 0 forward, 1 central, 2 backward.  The coefficients
+// are the reference's tables (:4596-4617) with the textbook signs of forward
+// orders 3 and 4 (the reference has +3/2 and +1/4 there, which makes the
+// coefficients not sum to zero); the steps are CVODE's difference-quotient
+// increments with inverse error weights w_i = 1 / (rtol |y_i| + atol), which
+// the reference's formula follows with w_i = rtol |y_i| + atol (its own test
+// marks that Jacobian broken, tests/test_jacobian.py:2683)
+static bool fd_table(int mode, int order, std::vector<int>& x, std::vector<double>& y) {
+    static const double fwd[7][7] = {
+        {0},
+        {-1, 1},
+        {-3.0 / 2, 2, -1.0 / 2},
+        {-11.0 / 6, 3, -3.0 / 2, 1.0 / 3},
+        {-25.0 / 12, 4, -3, 4.0 / 3, -1.0 / 4},
+        {-137.0 / 60, 5, -5, 10.0 / 3, -5.0 / 4, 1.0 / 5},
+        {-49.0 / 20, 6, -15.0 / 2, 20.0 / 3, -15.0 / 4, 6.0 / 5, -1.0 / 6}};
+    static const double cen[5][8] = {
+        {0},
+        {-0.5, 0.5},
+        {1.0 / 12, -2.0 / 3, 2.0 / 3, -1.0 / 12},
+        {-1.0 / 60, 3.0 / 20, -3.0 / 4, 3.0 / 4, -3.0 / 20, 1.0 / 60},
+        {1.0 / 280, -4.0 / 105, 1.0 / 5, -4.0 / 5, 4.0 / 5, -1.0 / 5, 4.0 / 105, -1.0 / 280}};
+    x.clear();
+    y.clear();
+    if (mode == 1) {
+        if (order < 2 || order > 8 || order % 2) return false;
+        for (int k = -order / 2; k <= order / 2; ++k)
+            if (k) x.push_back(k);
+        for (int k = 0; k < order; ++k) y.push_back(cen[order / 2][k]);
+        return true;
+    }
+    if ((mode != 0 && mode != 2) || order < 1 || order > 6) return false;
+    const double sgn = mode == 2 ? -1.0 : 1.0;
+    for (int k = 0; k <= order; ++k) {
+        x.push_back(mode == 2 ? -k : k);
+        y.push_back(sgn * fwd[order][k]);
+    }
+    return true;
+}
+
+int oracle_fd_jacobian(const pyjac_mech_tables* m, int conp, long n, const double* param,
+                       const double* phi, double* jac, int mode, int order, double rtol,
+                       double atol) {
+    if (!m) return PYJAC_ERR_ARG;
+    std::vector<int> xk;
+    std::vector<double> yk;
+    if (!fd_table(mode, order, xk, yk)) return PYJAC_ERR_ARG;
+    const int NN = m->n_species + 1;
+    const double eps = DBL_EPSILON;
+    Work<double> w;
+    w.init(*m);
+    std::vector<double> x(NN), f0(NN), f(NN), ewt(NN);
+    for (long s = 0; s < n; ++s) {
+        const double* p = phi + s * NN;
+        double* J = jac + s * (size_t)NN * NN;
+        for (int i = 0; i < NN; ++i) x[i] = p[i];
+        species_rates_state<double>(*m, conp, param[s], x.data(), f0.data(), w);
+        double sum = 0.0;
+        for (int i = 0; i < NN; ++i) {
+            ewt[i] = 1.0 / (atol + rtol * fabs(x[i]));
+            sum = sum + (ewt[i] * fabs(f0[i])) * (ewt[i] * fabs(f0[i]));
+        }
+        const double fac = sqrt(sum / (double)NN);
+        const double r0 = 1000.0 * rtol * eps * (double)NN * fac;
+        const double srur = sqrt(eps);
+        for (int i = 0; i < NN * NN; ++i) J[i] = 0.0;
+        for (int i = 0; i < NN; ++i) {
+            const double orig = x[i];
+            const double r = fmax(srur * fabs(orig), r0 / ewt[i]);
+            for (size_t k = 0; k < xk.size(); ++k) {
+                x[i] = orig + xk[k] * r;
+                species_rates_state<double>(*m, conp, param[s], x.data(), f.data(), w);
+                for (int j = 0; j < NN; ++j) J[j * NN + i] = J[j * NN + i] + yk[k] * f[j];
+            }
+            x[i] = orig;
+            for (int j = 0; j < NN; ++j) J[j * NN + i] = J[j * NN + i] / r;
         }
     }
     return PYJAC_OK;

@@ -22,6 +22,18 @@ hipError_t launch_jacobian(const DevMech& m, int conp, char order, long n, const
 hipError_t launch_chem_utils(const DevMech& m, int conp, char order, long n, const double* phi,
                              double* energy, double* heat, double* b, hipStream_t stream);
 
+// ---- finite-difference Jacobian (pyjac_fd.hip): C-order chunk buffers
+hipError_t launch_fd_gather(long cnt, int NN, long n, long s0, char order, const double* param,
+                            const double* phi, double* param_c, double* phi_c, hipStream_t stream);
+hipError_t launch_fd_steps(long cnt, int NN, const double* phi_c, const double* f0, double rtol,
+                           double atol, double* r, hipStream_t stream);
+hipError_t launch_fd_perturb(long cnt, int NN, int K, const int* xk, const double* param_c,
+                             const double* phi_c, const double* r, double* pparam, double* pphi,
+                             hipStream_t stream);
+hipError_t launch_fd_combine(long cnt, int NN, int K, const double* yk, const double* pf,
+                             const double* r, const int* acc_map, int nout, long n, long s0,
+                             char order, double* jac, hipStream_t stream);
+
 // ---- wave-per-state kernels (pyjac_wave.hip): per-state arrays in LDS
 struct WaveTables {
     const void* rx_i;  // int4 [Nr]: kind | (nu_sum + 128) << 8, rev, thd, n species

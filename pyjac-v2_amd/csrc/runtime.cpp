@@ -78,6 +78,13 @@ struct pyjac_handle {
     // LDS-kernel Jacobian in F order: C-order chunk before the transpose
     double* d_fchunk = nullptr;
     long fchunk_states = 0;
+    // finite-difference Jacobian: options, coefficients (device), buffers
+    int fd_mode = PYJAC_FD_FORWARD, fd_order = 1, fd_K = 0;
+    double fd_rtol = 1e-8, fd_atol = 1e-15;
+    int* d_fd_x = nullptr;
+    double* d_fd_y = nullptr;
+    double* d_fd = nullptr;
+    long fd_states = 0;
     // block kernels: device copies of their arguments, LDS layouts and the
     // tuned number of waves per state (0: default)
     pjk_host::BlockArgs* d_args[2] = {nullptr, nullptr};
@@ -205,6 +212,95 @@ int ensure_fchunk(pyjac_handle* h, long n) {
     return PYJAC_OK;
 }
 
+// Coefficients of the finite-difference Jacobian, the reference's tables
+// (core/create_jacobian.py:4596-4617) with the textbook signs of forward
+// orders 3 and 4 (the reference's +3/2 and +1/4 make those sums non-zero)
+bool fd_coefficients(int mode, int order, std::vector<int>& x, std::vector<double>& y) {
+    x.clear();
+    y.clear();
+    if (mode == PYJAC_FD_CENTRAL) {
+        switch (order) {
+            case 2: y = {-0.5, 0.5}; break;
+            case 4: y = {1.0 / 12, -2.0 / 3, 2.0 / 3, -1.0 / 12}; break;
+            case 6: y = {-1.0 / 60, 3.0 / 20, -3.0 / 4, 3.0 / 4, -3.0 / 20, 1.0 / 60}; break;
+            case 8:
+                y = {1.0 / 280, -4.0 / 105, 1.0 / 5, -4.0 / 5, 4.0 / 5, -1.0 / 5, 4.0 / 105,
+                     -1.0 / 280};
+                break;
+            default: return false;
+        }
+        for (int k = -order / 2; k <= order / 2; ++k)
+            if (k) x.push_back(k);
+        return true;
+    }
+    if (mode != PYJAC_FD_FORWARD && mode != PYJAC_FD_BACKWARD) return false;
+    switch (order) {
+        case 1: y = {-1.0, 1.0}; break;
+        case 2: y = {-3.0 / 2, 2.0, -1.0 / 2}; break;
+        case 3: y = {-11.0 / 6, 3.0, -3.0 / 2, 1.0 / 3}; break;
+        case 4: y = {-25.0 / 12, 4.0, -3.0, 4.0 / 3, -1.0 / 4}; break;
+        case 5: y = {-137.0 / 60, 5.0, -5.0, 10.0 / 3, -5.0 / 4, 1.0 / 5}; break;
+        case 6: y = {-49.0 / 20, 6.0, -15.0 / 2, 20.0 / 3, -15.0 / 4, 6.0 / 5, -1.0 / 6}; break;
+        default: return false;
+    }
+    for (int k = 0; k <= order; ++k) x.push_back(k);
+    if (mode == PYJAC_FD_BACKWARD) {
+        for (auto& v : x) v = -v;
+        for (auto& v : y) v = -v;
+    }
+    return true;
+}
+
+int fd_configure(pyjac_handle* h, int mode, int order, double rtol, double atol) {
+    std::vector<int> x;
+    std::vector<double> y;
+    if (!fd_coefficients(mode, order, x, y))
+        return fail(PYJAC_ERR_ARG, "no finite-difference coefficients for mode %d, order %d", mode,
+                    order);
+    if (!(rtol >= 0.0) || !(atol >= 0.0)) return fail(PYJAC_ERR_ARG, "negative tolerance");
+    if (!h->d_fd_x) {
+        HIP_TRY(hipMalloc(&h->d_fd_x, 8 * sizeof(int)));
+        HIP_TRY(hipMalloc(&h->d_fd_y, 8 * sizeof(double)));
+    }
+    HIP_TRY(hipMemcpy(h->d_fd_x, x.data(), x.size() * sizeof(int), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(h->d_fd_y, y.data(), y.size() * sizeof(double), hipMemcpyHostToDevice));
+    if ((int)x.size() != h->fd_K) {
+        // the buffers depend on the number of coefficients
+        (void)hipFree(h->d_fd);
+        h->d_fd = nullptr;
+        h->fd_states = 0;
+    }
+    h->fd_mode = mode;
+    h->fd_order = order;
+    h->fd_K = (int)x.size();
+    h->fd_rtol = rtol;
+    h->fd_atol = atol;
+    return PYJAC_OK;
+}
+
+// doubles per state of the finite-difference buffers: C-order state, base
+// rates, steps, and the NN * K perturbed states with their rates
+size_t fd_doubles_per_state(const pyjac_handle* h) {
+    const size_t NN = h->m.NN, K = h->fd_K;
+    return 1 + 3 * NN + NN * K + 2 * NN * K * NN;
+}
+
+long fd_chunk(const pyjac_handle* h, long n) {
+    long c = (long)(((size_t)768 << 20) / (fd_doubles_per_state(h) * sizeof(double))) / 64 * 64;
+    return std::max(64L, std::min(c, (n + 63) / 64 * 64));
+}
+
+int ensure_fd(pyjac_handle* h, long n) {
+    const long want = fd_chunk(h, n);
+    if (h->d_fd && h->fd_states >= want) return PYJAC_OK;
+    (void)hipFree(h->d_fd);
+    h->d_fd = nullptr;
+    h->fd_states = 0;
+    HIP_TRY(hipMalloc(&h->d_fd, (size_t)want * fd_doubles_per_state(h) * sizeof(double)));
+    h->fd_states = want;
+    return PYJAC_OK;
+}
+
 // chunk buffer of the device-memory variant (about 2 GiB): per state its
 // contributions and, in F order, its entries
 int ensure_chunk(pyjac_handle* h, long n) {
@@ -227,6 +323,15 @@ int ensure_chunk(pyjac_handle* h, long n) {
 // buffers), allocated ahead of a graph capture (no allocation may happen
 // while a stream is captured)
 int prepare_lane(pyjac_handle* h, int kernel, long n) {
+    if (kernel == 1 && h->jac_type == PYJAC_JAC_FINITE_DIFFERENCE) {
+        int rc = ensure_fd(h, n);
+        if (rc || use_block(h, 0)) return rc;
+        // the species-rate launches over the perturbed states
+        const long m = fd_chunk(h, n) * h->m.NN * h->fd_K;
+        size_t per = pjk_host::ws_doubles_per_state(h->m, false);
+        int grid = pick_grid(h, per, h->variant[0], m);
+        return ensure_ws(h, per * (size_t)grid * h->block * sizeof(double));
+    }
     if (kernel == 1 && use_glob(h)) return ensure_chunk(h, n);
     if (kernel == 1 && use_block(h, 1)) return ensure_fchunk(h, n);
     if (use_block(h, kernel)) return PYJAC_OK;
@@ -260,9 +365,56 @@ int run_rates(pyjac_handle* h, long n, const double* param, const double* phi, d
     return PYJAC_OK;
 }
 
+// species rates of n states in C order (the buffers of the finite-difference
+// Jacobian), whatever the handle's data order
+int run_rates_c(pyjac_handle* h, long n, const double* param, const double* phi, double* dphi,
+                hipStream_t s) {
+    if (use_block(h, 0)) {
+        HIP_TRY(pjk_host::launch_rates_block(h->m, h->d_args[0], h->lds[0], block_wps(h, 0),
+                                             h->conp, 'C', n, param, phi, dphi, s));
+        return PYJAC_OK;
+    }
+    size_t per = pjk_host::ws_doubles_per_state(h->m, false);
+    int grid = pick_grid(h, per, h->variant[0], n);
+    int rc = ensure_ws(h, per * (size_t)grid * h->block * sizeof(double));
+    if (rc) return rc;
+    HIP_TRY(pjk_host::launch_species_rates(h->m, h->conp, 'C', n, param, phi, dphi, h->d_ws, grid,
+                                           h->block, s));
+    return PYJAC_OK;
+}
+
+// finite-difference Jacobian (create_jacobian.py:4523-4896), in chunks
+int run_fd_jac(pyjac_handle* h, long n, const double* param, const double* phi, double* jac,
+               hipStream_t s) {
+    int rc = ensure_fd(h, n);
+    if (rc) return rc;
+    const long NN = h->m.NN, K = h->fd_K, cs = h->fd_states;
+    double* param_c = h->d_fd;
+    double* phi_c = param_c + cs;
+    double* f0 = phi_c + cs * NN;
+    double* r = f0 + cs * NN;
+    double* pparam = r + cs * NN;
+    double* pphi = pparam + cs * NN * K;
+    double* pf = pphi + cs * NN * K * NN;
+    for (long s0 = 0; s0 < n; s0 += cs) {
+        const long cnt = std::min(cs, n - s0);
+        HIP_TRY(pjk_host::launch_fd_gather(cnt, (int)NN, n, s0, h->order, param, phi, param_c,
+                                           phi_c, s));
+        if ((rc = run_rates_c(h, cnt, param_c, phi_c, f0, s))) return rc;
+        HIP_TRY(pjk_host::launch_fd_steps(cnt, (int)NN, phi_c, f0, h->fd_rtol, h->fd_atol, r, s));
+        HIP_TRY(pjk_host::launch_fd_perturb(cnt, (int)NN, (int)K, h->d_fd_x, param_c, phi_c, r,
+                                            pparam, pphi, s));
+        if ((rc = run_rates_c(h, cnt * NN * K, pparam, pphi, pf, s))) return rc;
+        HIP_TRY(pjk_host::launch_fd_combine(cnt, (int)NN, (int)K, h->d_fd_y, pf, r, h->bt.acc_map,
+                                            nvalues(h), n, s0, h->order, jac, s));
+    }
+    return PYJAC_OK;
+}
+
 int run_jac(pyjac_handle* h, long n, const double* param, const double* phi, double* jac,
             hipStream_t s) {
     if (n <= 0) return PYJAC_OK;
+    if (h->jac_type == PYJAC_JAC_FINITE_DIFFERENCE) return run_fd_jac(h, n, param, phi, jac, s);
     if (use_block(h, 1)) {
         int rc = ensure_fchunk(h, n);
         if (rc) return rc;
@@ -395,9 +547,10 @@ int pyjac_hip_create(const pyjac_mech_tables* mt, int conp, char order, int jac_
     if (order != 'C' && order != 'F') return fail(PYJAC_ERR_ARG, "order must be 'C' or 'F'");
     if (jac_format != PYJAC_JAC_FULL && jac_format != PYJAC_JAC_SPARSE)
         return fail(PYJAC_ERR_ARG, "bad jac_format %d", jac_format);
-    if (jac_type != PYJAC_JAC_EXACT)
+    if (jac_type != PYJAC_JAC_EXACT && jac_type != PYJAC_JAC_FINITE_DIFFERENCE)
         return fail(PYJAC_ERR_UNSUPPORTED,
-                    "jac_type %d not supported by the HIP backend (exact only)", jac_type);
+                    "jac_type %d not supported by the HIP backend (exact or finite difference)",
+                    jac_type);
     const int Ns = mt->n_species, Nr = mt->n_reactions, NN = Ns + 1;
     if (Ns < 2 || Nr < 1) return fail(PYJAC_ERR_ARG, "empty mechanism");
     if (mt->n_cheb && (mt->cheb_max_p > 16 || mt->cheb_max_t > 16))
@@ -792,8 +945,23 @@ int pyjac_hip_create(const pyjac_mech_tables* mt, int conp, char order, int jac_
             return fail(PYJAC_ERR_HIP, "argument upload failed: %s", hipGetErrorString(e2));
         }
     }
+    if (jac_type == PYJAC_JAC_FINITE_DIFFERENCE) {
+        // the reference's defaults: forward, first order, rtol 1e-8, atol 1e-15
+        int rc = fd_configure(h, PYJAC_FD_FORWARD, 1, 1e-8, 1e-15);
+        if (rc) {
+            pyjac_hip_destroy(h);
+            return rc;
+        }
+    }
     *out = h;
     return PYJAC_OK;
+}
+
+int pyjac_hip_set_fd_options(pyjac_handle* h, int mode, int order, double rtol, double atol) {
+    if (!h) return fail(PYJAC_ERR_ARG, "null handle");
+    if (h->jac_type != PYJAC_JAC_FINITE_DIFFERENCE)
+        return fail(PYJAC_ERR_STATE, "handle was not created with PYJAC_JAC_FINITE_DIFFERENCE");
+    return fd_configure(h, mode, order, rtol, atol);
 }
 
 int pyjac_hip_destroy(pyjac_handle* h) {
@@ -803,6 +971,9 @@ int pyjac_hip_destroy(pyjac_handle* h) {
     (void)hipFree(h->d_args_g);
     (void)hipFree(h->d_chunk);
     (void)hipFree(h->d_fchunk);
+    (void)hipFree(h->d_fd);
+    (void)hipFree(h->d_fd_x);
+    (void)hipFree(h->d_fd_y);
     (void)hipFree(h->d_blob);
     (void)hipFree(h->d_ws);
     (void)hipFree(h->d_param);

@@ -65,6 +65,7 @@ def load_library(path=None):
     L.pyjac_hip_batch_destroy.argtypes = [P]
     L.pyjac_hip_autotune.argtypes = [P, I, S, P, P, P, P, ctypes.POINTER(I),
                                      ctypes.POINTER(ctypes.c_float)]
+    L.pyjac_hip_set_fd_options.argtypes = [P, I, I, ctypes.c_double, ctypes.c_double]
     L.pyjac_hip_read_initial_conditions.argtypes = [ctypes.c_char_p, S, I, P, P, ctypes.c_char]
     L.pyjac_hip_write_data.argtypes = [ctypes.c_char_p, P, S]
     if path == LIB_PATH or _lib is None:
@@ -157,6 +158,15 @@ class _KernelBase(object):
         """Resize the staging buffers for ``problem_size`` states."""
         check(self.lib.pyjac_hip_resize(self._h, int(problem_size), int(work_size)))
         self.problem_size = int(problem_size)
+
+    def set_fd_options(self, mode=0, order=1, rtol=1e-8, atol=1e-15):
+        """Finite-difference Jacobian options (a kernel made with
+        ``jac_type=JacobianType.finite_difference``): mode 0 forward, 1
+        central, 2 backward (``FiniteDifferenceMode``), the order, and the
+        tolerances of the steps (finite_difference_jacobian,
+        core/create_jacobian.py:4523-4526)."""
+        check(self.lib.pyjac_hip_set_fd_options(self._h, int(mode), int(order), float(rtol),
+                                                float(atol)))
 
     def set_variant(self, variant):
         check(self.lib.pyjac_hip_set_variant(self._h, self._variant_slot, int(variant)))

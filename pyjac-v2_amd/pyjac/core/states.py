@@ -62,3 +62,27 @@ def write_data_bin(path, param, phi):
     with open(path, 'wb') as f:
         np.ascontiguousarray(rec).tofile(f)
     return path
+
+
+def states_from_mass_fractions(mech, T, P, Y, species, conp=True, V=1.0):
+    """``(param, phi)`` (C order) of states given by temperature [K],
+    pressure [Pa] and mass fractions ``Y[n, len(species)]`` of the named
+    species (the others zero), at volume V [m^3]: moles n_k = X_k P V / (RU T)
+    with X from Y and the molecular weights.  Every named species must be in
+    the mechanism."""
+    T = np.asarray(T, dtype=np.float64).reshape(-1)
+    P = np.asarray(P, dtype=np.float64).reshape(-1)
+    Y = np.asarray(Y, dtype=np.float64).reshape(len(T), -1)
+    names = list(mech.species_names)
+    full = np.zeros((len(T), mech.Ns))
+    for c, name in enumerate(species):
+        full[:, names.index(str(name))] = Y[:, c]
+    X = full / mech.mw[None, :]
+    X /= X.sum(axis=1, keepdims=True)
+    ntot = P * V / (RU * T)
+    phi = np.empty((len(T), mech.NN))
+    phi[:, 0] = T
+    phi[:, 1] = V if conp else P
+    phi[:, 2:] = (X * ntot[:, None])[:, :mech.Ns - 1]
+    param = P.copy() if conp else np.full(len(T), V, dtype=np.float64)
+    return param, phi

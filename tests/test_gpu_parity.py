@@ -135,3 +135,30 @@ def test_chunked_f_order(mechs, monkeypatch):
     b = jref.reshape(-1, n).T
     fro = np.linalg.norm(a - b, axis=1) / np.linalg.norm(b, axis=1)
     assert fro.max() <= JAC_FRO, fro.max()
+
+
+@pytest.mark.parametrize('conp,order,fmt', [(True, 'C', 0), (False, 'F', 1)])
+def test_pasr_states(mechs, conp, order, fmt):
+    """The reference's partially stirred reactor states (H2/air near
+    equilibrium, mapped into GRI-Mech 3.0): rates and Jacobians, where the
+    production rates are sums of nearly cancelling forward and reverse
+    rates."""
+    from test_pasr_states import pasr_states
+    mech = mechs('gri30')
+    param, phi = pasr_states(mech, conp)
+    n = len(param)
+    phi_in = phi if order == 'C' else np.ascontiguousarray(phi.T)
+    rk = PySpeciesRatesKernel(n, 0, mech=mech, conp=conp, order=order)
+    out = np.zeros_like(phi_in)
+    rk(param, phi_in, out)
+    got = out if order == 'C' else out.T
+    assert np.all(np.isfinite(got))
+    assert _rate_err(got, mech, conp, param, phi) <= 1.0
+    jref = oracle.to_layout(mech, oracle.jacobian(mech, conp, param, phi), order, fmt)
+    jk = PyJacobianKernel(n, 0, mech=mech, conp=conp, order=order, jac_format=fmt)
+    jac = np.zeros_like(jref)
+    jk(param, phi_in, jac)
+    a = jac.reshape(n, -1) if order == 'C' else jac.reshape(-1, n).T
+    b = jref.reshape(n, -1) if order == 'C' else jref.reshape(-1, n).T
+    fro = np.linalg.norm(a - b, axis=1) / np.linalg.norm(b, axis=1)
+    assert fro.max() <= JAC_FRO, fro.max()

@@ -39,6 +39,7 @@ def lib():
         L.oracle_chem_utils.argtypes = [P, ctypes.c_int, ctypes.c_long, P, P, P, P]
         L.oracle_jacobian.argtypes = [P, ctypes.c_int, ctypes.c_long, P, P, P, ctypes.c_int]
         L.oracle_jacobian_ad.argtypes = [P, ctypes.c_int, ctypes.c_long, P, P, P]
+        L.oracle_jacobian_masked.argtypes = [P, ctypes.c_int, ctypes.c_long, P, P, P, P]
         L.oracle_fd_jacobian.argtypes = [P, ctypes.c_int, ctypes.c_long, P, P, P, ctypes.c_int,
                                          ctypes.c_int, ctypes.c_double, ctypes.c_double]
         _lib = L
@@ -105,6 +106,23 @@ def jacobian_ad(mech, conp, param, phi):
     out = np.empty((n, mech.NN, mech.NN))
     st = lib().oracle_jacobian_ad(ctypes.cast(mech.pointer(), ctypes.c_void_p), int(conp), n,
                                   _p(param), _p(phi), _p(out))
+    assert st == 0, st
+    return out
+
+
+def jacobian_on_pattern(mech, conp, param, phi):
+    """Dense analytical Jacobian ``[n, NN, NN]`` on the mechanism's sparse
+    pattern (``mech.sparse_rows_cols``): the T and extra-variable rows sum
+    only the stored species entries.  With an approximate pattern, this is
+    the approximate Jacobian."""
+    param, phi = _prep(mech, param, phi)
+    n = phi.shape[0]
+    rows, cols = mech.sparse_rows_cols('C')
+    mask = np.zeros((mech.NN, mech.NN), dtype=np.uint8)
+    mask[rows, cols] = 1
+    out = np.empty((n, mech.NN, mech.NN))
+    st = lib().oracle_jacobian_masked(ctypes.cast(mech.pointer(), ctypes.c_void_p), int(conp), n,
+                                      _p(param), _p(phi), _p(out), _p(mask))
     assert st == 0, st
     return out
 

@@ -539,6 +539,11 @@ struct JacCtx {
     std::vector<double> cp_like, energy, dspec, db;
     double spec_heat_tot;
     double& at(int row, int col) { return J[row * NN + col]; }
+    // stored entries of a sparse pattern (null: every entry): the T and
+    // extra-variable rows sum only the stored species entries, as the
+    // reference's conditional Jacobian reads do (jac_create)
+    const unsigned char* mask = nullptr;
+    bool stored(int row, int col) const { return !mask || mask[row * NN + col]; }
 };
 
 inline double conc_g(const JacCtx& c, int k) { return fmax(SMALL, c.w.conc[k]); }
@@ -724,15 +729,16 @@ void rows_TE_dnj(JacCtx& c) {
     for (int j = 0; j < c.Ns - 1; ++j) {
         double sum = 0.0;
         for (int k = 0; k < c.Ns - 1; ++k)
-            sum = sum + (c.w.energy[k] + -1.0 * c.w.energy[ns] * m.mw_post[k]) *
-                            c.at(k + 2, j + 2);
+            if (c.stored(k + 2, j + 2))
+                sum = sum + (c.w.energy[k] + -1.0 * c.w.energy[ns] * m.mw_post[k]) *
+                                c.at(k + 2, j + 2);
         c.at(0, j + 2) = -1.0 * (sum + c.dTdt * (c.w.spec_heat[j] + -1.0 * c.w.spec_heat[ns])) /
                          (c.V * c.spec_heat_tot);
     }
     for (int j = 0; j < c.Ns - 1; ++j) {
         double sum = 0.0;
         for (int k = 0; k < c.Ns - 1; ++k)
-            sum = sum + (1.0 + -1.0 * m.mw_post[k]) * c.at(k + 2, j + 2);
+            if (c.stored(k + 2, j + 2)) sum = sum + (1.0 + -1.0 * m.mw_post[k]) * c.at(k + 2, j + 2);
         c.at(1, j + 2) = c.at(1, j + 2) + c.T * RU * sum / fixed + extra * c.at(0, j + 2) * (1.0 / c.T);
     }
 }
@@ -1164,11 +1170,12 @@ void column_E(JacCtx& c) {
 
 void jacobian_state(const pyjac_mech_tables& m, int conp, double param,
                     const double* phi, double* jac, Work<double>& w,
-                    std::vector<double>& dphi) {
+                    std::vector<double>& dphi, const unsigned char* mask = nullptr) {
     const int Ns = m.n_species, NN = Ns + 1;
     species_rates_state<double>(m, conp, param, phi, dphi.data(), w);
     std::fill(jac, jac + (size_t)NN * NN, 0.0);
     JacCtx c{m, conp, Ns, NN, Ns - 1, jac, w};
+    c.mask = mask;
     c.T = phi[0];
     c.P = conp ? param : phi[1];
     c.V = conp ? phi[1] : param;
@@ -1285,6 +1292,22 @@ int oracle_jacobian(const pyjac_mech_tables* m, int conp, long n,
         }
     } fn{m, conp, NN, param, phi, jac};
     parallel_states(n, nthreads, fn);
+    return PYJAC_OK;
+}
+
+// analytical Jacobian on a sparse pattern (mask[NN][NN], 1 = stored): the
+// approximate Jacobian's pattern (determine_jac_inds, create_jacobian.py:187)
+// keeps the exact values of the stored entries, and the T and extra-variable
+// rows sum only those
+int oracle_jacobian_masked(const pyjac_mech_tables* m, int conp, long n, const double* param,
+                           const double* phi, double* jac, const unsigned char* mask) {
+    if (!m || !param || !phi || !jac || !mask) return PYJAC_ERR_ARG;
+    const int NN = m->n_species + 1;
+    Work<double> w;
+    w.init(*m);
+    std::vector<double> dphi(NN, 0.0);
+    for (long s = 0; s < n; ++s)
+        jacobian_state(*m, conp, param[s], phi + s * NN, jac + s * (size_t)NN * NN, w, dphi, mask);
     return PYJAC_OK;
 }
 

@@ -78,6 +78,9 @@ struct BlockTables {
     const void* gcombo;      // int4 [n_combo]: pair, target (as rec_pos), efficiency
     const int* ent_rc;       // [jac_nnz]: (row << 16) | column of each output entry
     int n_contrib, n_long;
+    // approximate Jacobian on a sparse pattern: rows T and V|P sum only the
+    // stored species entries, bath-gas sums included
+    int approx_sums;
 };
 
 // LDS offsets (in doubles) of one workgroup's arrays

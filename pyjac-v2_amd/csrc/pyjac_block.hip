@@ -1171,6 +1171,7 @@ __device__ __forceinline__ void jacobian_global_state(const BlockArgs* __restric
     lds_double* col1 = col0 + NN;
     lds_double* lng = col1 + NN;       // entries with many contributions
     const JacAcc acc{nullptr, cb, col0, col1, lng};
+    const bool approx = c.bt.approx_sums != 0;
     if (tid == 0) { c.conc(Ns) = 1.0; c.b(Ns) = 0.0; c.db(Ns) = 0.0; }
     for (int k = tid; k < Ns - 1; k += NT) c.mw(k) = c.m.mw_post[k];
     for (int p = tid; p < c.bt.n_pairs; p += NT) c.Cp(p) = 0.0;
@@ -1271,9 +1272,11 @@ __device__ __forceinline__ void jacobian_global_state(const BlockArgs* __restric
                         for (int i = 1; i < GC; ++i) v += x[u][i];
                     }
                     const double mw = c.mw(k);
-                    lds_add(cs0 + col - 2, (c.energy(k) + -1.0 * e_ns * mw) * v);
-                    lds_add(cs1 + col - 2, (1.0 + -1.0 * mw) * v);
-                    v += c.U(k);
+                    const double vu = v + c.U(k);
+                    const double vs = approx ? vu : v;
+                    lds_add(cs0 + col - 2, (c.energy(k) + -1.0 * e_ns * mw) * vs);
+                    lds_add(cs1 + col - 2, (1.0 + -1.0 * mw) * vs);
+                    v = vu;
                 } else {
                     v = col == 0 ? col0[k] : col1[k];
                 }
@@ -1281,9 +1284,10 @@ __device__ __forceinline__ void jacobian_global_state(const BlockArgs* __restric
             }
         }
     }
-    // the bath-gas sums join every species column of their row
+    // the bath-gas sums join every species column of their row (with an
+    // approximate pattern only the stored entries, summed above)
     double su0 = 0.0, su1 = 0.0;
-    for (int k = lane; k < Ns - 1; k += 64) {
+    for (int k = lane; k < Ns - 1 && !approx; k += 64) {
         const double u = c.U(k), mw = c.mw(k);
         su0 += (c.energy(k) + -1.0 * e_ns * mw) * u;
         su1 += (1.0 + -1.0 * mw) * u;

@@ -547,10 +547,9 @@ int pyjac_hip_create(const pyjac_mech_tables* mt, int conp, char order, int jac_
     if (order != 'C' && order != 'F') return fail(PYJAC_ERR_ARG, "order must be 'C' or 'F'");
     if (jac_format != PYJAC_JAC_FULL && jac_format != PYJAC_JAC_SPARSE)
         return fail(PYJAC_ERR_ARG, "bad jac_format %d", jac_format);
-    if (jac_type != PYJAC_JAC_EXACT && jac_type != PYJAC_JAC_FINITE_DIFFERENCE)
-        return fail(PYJAC_ERR_UNSUPPORTED,
-                    "jac_type %d not supported by the HIP backend (exact or finite difference)",
-                    jac_type);
+    if (jac_type != PYJAC_JAC_EXACT && jac_type != PYJAC_JAC_APPROXIMATE &&
+        jac_type != PYJAC_JAC_FINITE_DIFFERENCE)
+        return fail(PYJAC_ERR_ARG, "bad jac_type %d", jac_type);
     const int Ns = mt->n_species, Nr = mt->n_reactions, NN = Ns + 1;
     if (Ns < 2 || Nr < 1) return fail(PYJAC_ERR_ARG, "empty mechanism");
     if (mt->n_cheb && (mt->cheb_max_p > 16 || mt->cheb_max_t > 16))
@@ -924,6 +923,19 @@ int pyjac_hip_create(const pyjac_mech_tables* mt, int conp, char order, int jac_
     h->block_ok_jac = packable && pjk_host::block_lds_bytes(m, true, h->bt.n_pairs) <= LDS_PER_CU;
     h->glob_ok_jac = packable && glob_tables &&
                      pjk_host::block_lds_bytes(m, true, h->bt.n_pairs, false, n_long) <= LDS_PER_CU;
+    // an approximate Jacobian on a sparse pattern: only the kernels that sum
+    // rows T and V|P over the stored entries compute it
+    h->bt.approx_sums = jac_type == PYJAC_JAC_APPROXIMATE && jac_format == PYJAC_JAC_SPARSE;
+    if (h->bt.approx_sums) {
+        h->block_ok_jac = false;
+        h->wave_ok_jac = false;
+        if (!h->glob_ok_jac) {
+            pyjac_hip_destroy(h);
+            return fail(PYJAC_ERR_UNSUPPORTED,
+                        "approximate sparse Jacobian: the mechanism does not fit the kernels "
+                        "that compute it");
+        }
+    }
     if (getenv("PYJAC_HIP_VERBOSE"))
         fprintf(stderr,
                 "pyjac_hip: Ns %d Nr %d nnz %d | LDS bytes: rates %zu, Jacobian %zu (dense), "
@@ -1213,7 +1225,7 @@ int pyjac_hip_autotune(pyjac_handle* h, int kernel, size_t n, const double* d_pa
     if (kernel == 1 && h->glob_ok_jac)
         for (int w : {2, 4, 8}) cand.push_back(20 + w);
     if (wave) cand.push_back(2);
-    cand.push_back(1);
+    if (!(kernel == 1 && h->bt.approx_sums)) cand.push_back(1);
     const int saved_variant = h->variant[kernel], saved_wps = h->wps[kernel];
     const int saved_wps_g = h->wps_g;
     hipEvent_t e0, e1;
@@ -1271,7 +1283,8 @@ int pyjac_hip_set_variant(pyjac_handle* h, int kernel, int variant) {
         const bool block = kernel == 0 ? h->block_ok_rates : h->block_ok_jac;
         const int w = variant % 10;
         const bool wok = w == 1 || w == 2 || w == 4 || w == 8;
-        bool ok = variant == 0 || variant == 1 || (variant == 2 && wave) ||
+        const bool lane_ok = !(kernel == 1 && h->bt.approx_sums);
+        bool ok = variant == 0 || (variant == 1 && lane_ok) || (variant == 2 && wave) ||
                   (variant > 10 && variant < 20 && wok && block) ||
                   (kernel == 1 && variant > 20 && variant < 30 && wok && h->glob_ok_jac);
         if (!ok)

@@ -20,7 +20,8 @@ import logging
 import os
 import shutil
 
-from .enum_types import JacobianFormat, JacobianType, KernelType, RateSpecialization
+from .enum_types import (FiniteDifferenceMode, JacobianFormat, JacobianType, KernelType,
+                         RateSpecialization)
 from .exceptions import InvalidInputSpecificationException
 from .mech_tables import MechTables
 
@@ -45,7 +46,8 @@ mech = _MechTables.from_file(
     _os.path.join(_here, _opts['thermo']) if _opts['thermo'] else None,
     last_spec=_opts['last_spec'], rate_spec=_opts['rate_spec'], jac_type=_opts['jac_type'])
 globals().update(_bind(mech, conp=_opts['conp'], order=_opts['order'],
-                       jac_format=_opts['jac_format'], jac_type=_opts['jac_type']))
+                       jac_format=_opts['jac_format'], jac_type=_opts['jac_type'],
+                       fd_mode=_opts.get('fd_mode', 0), fd_order=_opts.get('fd_order', 1)))
 species_names = list(mech.species_names)
 reaction_strings = list(mech.reaction_strings)
 '''
@@ -55,6 +57,7 @@ def create_jacobian(lang, mech_name=None, therm_name=None, build_path='./out/',
                     last_spec=None, kernel_type=KernelType.jacobian, data_order='C',
                     rate_specialization=RateSpecialization.hybrid, conp=True,
                     jac_type=JacobianType.exact, jac_format=JacobianFormat.sparse,
+                    fd_order=1, fd_mode=FiniteDifferenceMode.forward,
                     build_library=True, **ignored):
     """Write the HIP kernel module for one mechanism; returns its path."""
     if lang != 'hip':
@@ -64,8 +67,6 @@ def create_jacobian(lang, mech_name=None, therm_name=None, build_path='./out/',
     if data_order not in ('C', 'F'):
         raise InvalidInputSpecificationException('data_order')
     jac_type = JacobianType(jac_type)
-    if jac_type != JacobianType.exact:
-        raise NotImplementedError('the HIP target computes exact Jacobians only')
     # parse and build the tables now, so that a bad mechanism fails here
     mech = MechTables.from_file(mech_name, therm_name, last_spec=last_spec,
                                 rate_spec=rate_specialization, jac_type=jac_type)
@@ -79,6 +80,7 @@ def create_jacobian(lang, mech_name=None, therm_name=None, build_path='./out/',
     opts = dict(mech=mech_base, thermo=therm_base, last_spec=last_spec,
                 rate_spec=int(RateSpecialization(rate_specialization)),
                 jac_type=int(jac_type), jac_format=int(JacobianFormat(jac_format)),
+                fd_mode=int(FiniteDifferenceMode(fd_mode)), fd_order=int(fd_order),
                 conp=bool(conp), order=data_order, kernel_type=KernelType(kernel_type).name)
     with open(os.path.join(build_path, 'pyjac_hip.json'), 'w') as f:
         json.dump(opts, f, indent=1)

@@ -118,6 +118,12 @@ class _KernelBase(object):
         self.order = order
         self.jac_format = int(JacobianFormat(jac_format))
         self.jac_type = int(JacobianType(jac_type))
+        if (self.jac_type == JacobianType.approximate) != \
+                (getattr(mech, 'jac_type', JacobianType.exact) == JacobianType.approximate):
+            # the pattern belongs to the tables (determine_jac_inds)
+            raise InvalidInputSpecificationException(
+                'jac_type: an approximate Jacobian needs tables built with '
+                'MechTables(..., jac_type=JacobianType.approximate), and only then')
         h = ctypes.c_void_p()
         check(self.lib.pyjac_hip_create(ctypes.cast(mech.pointer(), ctypes.c_void_p),
                                         int(self.conp), order.encode(),
@@ -385,7 +391,7 @@ PyChem_UtilsKernel = PyChemUtilsKernel
 
 
 def bind(mech, conp=True, order='C', jac_format=JacobianFormat.sparse,
-         jac_type=JacobianType.exact):
+         jac_type=JacobianType.exact, fd_mode=0, fd_order=1):
     """Kernel classes with the reference's constructor signature
     ``(problem_size, work_size, do_not_compile=False)`` bound to one
     mechanism and option set -- what the reference's generated
@@ -399,6 +405,9 @@ def bind(mech, conp=True, order='C', jac_format=JacobianFormat.sparse,
                 kw2 = dict(opts)
                 kw2.update(kw)
                 base.__init__(self, problem_size, work_size, do_not_compile, **kw2)
+                if base is PyJacobianKernel and \
+                        JacobianType(kw2['jac_type']) == JacobianType.finite_difference:
+                    self.set_fd_options(fd_mode, fd_order)
         Bound.__name__ = base.__name__
         return Bound
 

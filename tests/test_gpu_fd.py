@@ -62,3 +62,26 @@ def test_fd_options_need_a_fd_handle(mechs):
                           jac_type=JacobianType.finite_difference)
     with pytest.raises(HipRuntimeError):
         kf.set_fd_options(1, 3)  # central differences have even orders only
+
+
+@pytest.mark.parametrize('name,conp,order', [('gri30', True, 'C'), ('gri30', False, 'F'),
+                                             ('test', True, 'F'), ('s150', False, 'C')])
+def test_approximate_jacobian(mechs, name, conp, order):
+    """jac_type approximate on its sparser pattern (tables built with
+    JacobianType.approximate) against the checker's Jacobian on that pattern
+    (tests/test_approximate.py pins the checker)."""
+    from pyjac._paths import mechanism_files
+    from pyjac.core.mech_tables import MechTables
+    mech = MechTables.from_file(*mechanism_files(name), jac_type=JacobianType.approximate)
+    n = 300
+    param, phi = sample_states(mech, n, seed=41, conp=conp)
+    phi_in = phi if order == 'C' else np.ascontiguousarray(phi.T)
+    ref = oracle.to_layout(mech, oracle.jacobian_on_pattern(mech, conp, param, phi), order, 1)
+    k = PyJacobianKernel(n, 0, mech=mech, conp=conp, order=order, jac_format=1,
+                         jac_type=JacobianType.approximate)
+    out = np.zeros_like(ref)
+    k(param, phi_in, out)
+    a = out.reshape(n, -1) if order == 'C' else out.reshape(-1, n).T
+    b = ref.reshape(n, -1) if order == 'C' else ref.reshape(-1, n).T
+    fro = np.linalg.norm(a - b, axis=1) / np.linalg.norm(b, axis=1)
+    assert fro.max() <= 1e-8, fro.max()

@@ -40,6 +40,9 @@ def lib():
         L.oracle_jacobian.argtypes = [P, ctypes.c_int, ctypes.c_long, P, P, P, ctypes.c_int]
         L.oracle_jacobian_ad.argtypes = [P, ctypes.c_int, ctypes.c_long, P, P, P]
         L.oracle_jacobian_masked.argtypes = [P, ctypes.c_int, ctypes.c_long, P, P, P, P]
+        L.oracle_integrate_ros2.argtypes = [P, ctypes.c_int, ctypes.c_long, P, P, ctypes.c_double,
+                                            ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                            ctypes.c_int, ctypes.c_long, P]
         L.oracle_fd_jacobian.argtypes = [P, ctypes.c_int, ctypes.c_long, P, P, P, ctypes.c_int,
                                          ctypes.c_int, ctypes.c_double, ctypes.c_double]
         _lib = L
@@ -138,6 +141,22 @@ def fd_jacobian(mech, conp, param, phi, mode=0, order=1, rtol=1e-8, atol=1e-15):
                                   float(rtol), float(atol))
     assert st == 0, st
     return out
+
+
+def integrate_ros2(mech, conp, param, phi, t_end, rtol=1e-6, atol=1e-12, h0=0.0, fixed=False,
+                   max_steps=100000):
+    """ROS2 integration (the scheme of include/pyjac_integrator.h) of each
+    state over [0, t_end]; returns (phi at t_end, stats [accepted, rejected,
+    largest attempts, failed])."""
+    param, phi = _prep(mech, param, phi)
+    y = phi.copy()
+    stats = np.zeros(4, dtype=np.int64)
+    st = lib().oracle_integrate_ros2(ctypes.cast(mech.pointer(), ctypes.c_void_p), int(conp),
+                                     y.shape[0], _p(param), _p(y), float(t_end), float(rtol),
+                                     float(atol), float(h0), int(bool(fixed)), int(max_steps),
+                                     _p(stats))
+    assert st == 0, st
+    return y, stats
 
 
 def to_layout(mech, jac_dense, order, jac_format):

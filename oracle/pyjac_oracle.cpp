@@ -1418,6 +1418,113 @@ int oracle_fd_jacobian(const pyjac_mech_tables* m, int conp, long n, const doubl
     return PYJAC_OK;
 }
 
+// ROS2 integration of the species-rate ODE (the scheme of
+// include/pyjac_integrator.h: two-stage L-stable Rosenbrock, gamma =
+// 1 + 1/sqrt(2), Verwer et al. 1999), per state, with the analytical
+// Jacobian and a dense LU with partial pivoting.  phi [n][NN] in place;
+// stats[4]: accepted, rejected, largest attempts of a state, failed states.
+static void lu_factor(std::vector<double>& A, std::vector<int>& piv, int N) {
+    for (int k = 0; k < N; ++k) {
+        int p = k;
+        double best = -1.0;
+        for (int i = k; i < N; ++i)
+            if (fabs(A[i * N + k]) > best) { best = fabs(A[i * N + k]); p = i; }
+        piv[k] = p;
+        if (p != k)
+            for (int j = 0; j < N; ++j) std::swap(A[k * N + j], A[p * N + j]);
+        for (int i = k + 1; i < N; ++i) A[i * N + k] /= A[k * N + k];
+        for (int i = k + 1; i < N; ++i)
+            for (int j = k + 1; j < N; ++j) A[i * N + j] -= A[i * N + k] * A[k * N + j];
+    }
+}
+
+static void lu_solve(const std::vector<double>& A, const std::vector<int>& piv, int N,
+                     std::vector<double>& b) {
+    for (int k = 0; k < N; ++k)
+        if (piv[k] != k) std::swap(b[k], b[piv[k]]);
+    for (int k = 0; k < N; ++k)
+        for (int i = k + 1; i < N; ++i) b[i] -= A[i * N + k] * b[k];
+    for (int k = N - 1; k >= 0; --k) {
+        b[k] /= A[k * N + k];
+        for (int i = 0; i < k; ++i) b[i] -= A[i * N + k] * b[k];
+    }
+}
+
+int oracle_integrate_ros2(const pyjac_mech_tables* m, int conp, long n, const double* param,
+                          double* phi, double t_end, double rtol, double atol, double h0,
+                          int fixed, long max_steps, long* stats) {
+    if (!m || !param || !phi || !stats) return PYJAC_ERR_ARG;
+    const int N = m->n_species + 1;
+    const double gamma = 1.7071067811865475244;
+    Work<double> w;
+    w.init(*m);
+    std::vector<double> y(N), f0(N), f1(N), k1(N), k2(N), y1(N), yn(N), J((size_t)N * N),
+        A((size_t)N * N), dphi(N);
+    std::vector<int> piv(N);
+    for (int i = 0; i < 4; ++i) stats[i] = 0;
+    for (long s = 0; s < n; ++s) {
+        double* ys = phi + s * N;
+        for (int i = 0; i < N; ++i) y[i] = ys[i];
+        double t = 0.0, h = h0 > 0.0 ? fmin(h0, t_end) : 0.0;
+        bool rejlast = false, done = !(t_end > 0.0);
+        long att = 0;
+        while (!done && att < max_steps) {
+            species_rates_state<double>(*m, conp, param[s], y.data(), f0.data(), w);
+            jacobian_state(*m, conp, param[s], y.data(), J.data(), w, dphi);
+            if (h <= 0.0) {
+                double r = 0.0;
+                for (int i = 0; i < N; ++i) r = fmax(r, fabs(f0[i]) / (atol + rtol * fabs(y[i])));
+                h = r > 0.0 ? 1e-2 / r : t_end;
+                h = fmin(h, t_end - t);
+            }
+            for (int i = 0; i < N; ++i)
+                for (int j = 0; j < N; ++j)
+                    A[i * N + j] = -gamma * h * J[i * N + j] + (i == j ? 1.0 : 0.0);
+            lu_factor(A, piv, N);
+            k1 = f0;
+            lu_solve(A, piv, N, k1);
+            for (int i = 0; i < N; ++i) y1[i] = y[i] + h * k1[i];
+            species_rates_state<double>(*m, conp, param[s], y1.data(), f1.data(), w);
+            for (int i = 0; i < N; ++i) k2[i] = f1[i] - 2.0 * k1[i];
+            lu_solve(A, piv, N, k2);
+            double sq = 0.0;
+            for (int i = 0; i < N; ++i) {
+                yn[i] = y[i] + 1.5 * h * k1[i] + 0.5 * h * k2[i];
+                const double e = 0.5 * h * (k1[i] + k2[i]);
+                const double wt = atol + rtol * fmax(fabs(y[i]), fabs(yn[i]));
+                sq += (e / wt) * (e / wt);
+            }
+            const double err = sqrt(sq / N);
+            const bool finite = err == err && err < 1e300;
+            const bool accept = fixed ? finite : (finite && err <= 1.0);
+            ++att;
+            double hn = h;
+            if (accept) {
+                y = yn;
+                t += h;
+                ++stats[0];
+                if (!fixed) hn = h * fmin(rejlast ? 1.0 : 5.0, fmax(0.2, 0.9 / sqrt(fmax(err, 1e-10))));
+                rejlast = false;
+            } else {
+                ++stats[1];
+                if (!fixed) hn = h * (finite ? fmax(0.2, 0.9 / sqrt(err)) : 0.2);
+                rejlast = true;
+            }
+            if (t_end - t <= 1e-12 * t_end) {
+                done = true;
+                t = t_end;
+            } else {
+                hn = fmin(hn, t_end - t);
+            }
+            h = hn;
+        }
+        stats[2] = std::max(stats[2], att);
+        if (!done) ++stats[3];
+        for (int i = 0; i < N; ++i) ys[i] = y[i];
+    }
+    return PYJAC_OK;
+}
+
 int oracle_abi_version(void) { return PYJAC_MECH_ABI_VERSION; }
 
 }  // extern "C"

@@ -19,7 +19,8 @@ LIB_NAME = 'libpyjac_hip.so'
 LIB_PATH = os.path.join(PKG_DIR, LIB_NAME)
 
 SOURCES = ['pyjac_kernels.hip', 'pyjac_wave.hip', 'pyjac_block.hip', 'pyjac_fd.hip',
-           'surrogate_kernels.hip', 'runtime.cpp', 'surrogate_runtime.cpp']
+           'pyjac_integrator.hip', 'surrogate_kernels.hip', 'runtime.cpp',
+           'surrogate_runtime.cpp']
 HEADERS = ['device_mech.h', 'kernels.h']
 
 # every HIP translation unit is compiled with packed FP32 disabled

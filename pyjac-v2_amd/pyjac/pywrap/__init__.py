@@ -416,3 +416,57 @@ def bind(mech, conp=True, order='C', jac_format=JacobianFormat.sparse,
             'PySpecies_RatesKernel': make(PySpeciesRatesKernel),
             'PyChemUtilsKernel': make(PyChemUtilsKernel),
             'PyChem_UtilsKernel': make(PyChemUtilsKernel)}
+
+
+class IntegrateStats(ctypes.Structure):
+    _fields_ = [('accepted', ctypes.c_long), ('rejected', ctypes.c_long),
+                ('max_steps', ctypes.c_long), ('failed', ctypes.c_long),
+                ('launches', ctypes.c_long)]
+
+
+class PyIntegrator(object):
+    """Batched implicit (ROS2) integration of the species-rate ODE on the GPU
+    (include/pyjac_integrator.h): every state of ``phi`` (C order ``[n, NN]``,
+    a device tensor, updated in place) over ``[0, t_end]`` with its own
+    adaptive step.  No counterpart in the reference, whose Jacobian exists for
+    such integrators."""
+
+    def __init__(self, mech, conp=True):
+        self.lib = load_library()
+        L = self.lib
+        P = ctypes.c_void_p
+        L.pyjac_hip_integrator_create.argtypes = [P, ctypes.c_int, ctypes.POINTER(P)]
+        L.pyjac_hip_integrator_destroy.argtypes = [P]
+        L.pyjac_hip_integrate_dev.argtypes = [P, ctypes.c_size_t, P, P, ctypes.c_double,
+                                              ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                              ctypes.c_int, ctypes.c_long, P,
+                                              ctypes.POINTER(IntegrateStats)]
+        self.mech = mech
+        self.conp = bool(conp)
+        h = ctypes.c_void_p()
+        check(L.pyjac_hip_integrator_create(ctypes.cast(mech.pointer(), ctypes.c_void_p),
+                                            int(self.conp), ctypes.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, '_h', None)
+        if h is not None and h.value:
+            self.lib.pyjac_hip_integrator_destroy(h)
+            self._h = None
+
+    def integrate(self, param, phi, t_end, rtol=1e-6, atol=1e-12, h0=0.0, fixed=False,
+                  max_steps=1000000):
+        """Integrate in place; returns the step statistics as a dict."""
+        pp, dp = _ptr(param)
+        ph, dph = _ptr(phi)
+        if not (dp and dph):
+            raise ValueError('integrate needs device arrays')
+        n = _numel(param)
+        if _numel(phi) != n * self.mech.NN:
+            raise ValueError('phi must hold n * NN values (C order)')
+        st = IntegrateStats()
+        check(self.lib.pyjac_hip_integrate_dev(self._h, n, pp, ph, float(t_end), float(rtol),
+                                               float(atol), float(h0), int(bool(fixed)),
+                                               int(max_steps), _current_stream_ptr(),
+                                               ctypes.byref(st)))
+        return {f: getattr(st, f) for f, _ in IntegrateStats._fields_}

@@ -142,6 +142,20 @@ typedef struct pyjac_mech_tables {
 
 typedef struct pyjac_handle pyjac_handle;
 
+/*
+ * Tables from a file (no counterpart in the reference, whose generated code
+ * holds them as constants): the binary written by pyJac's
+ * MechTables.save_binary -- magic "PJHIPTB1", int32 number of scalar fields
+ * and their int32 values in struct order, int32 number of arrays, then per
+ * array in struct order: uint8 type (0 int32, 1 float64), int64 length,
+ * data.  Lets C and C++ callers use the library without Python.  *out is
+ * owned by the caller and released with pyjac_hip_tables_free.
+ * PYJAC_ERR_IO: missing or short file; PYJAC_ERR_ARG: not such a file, or
+ * another table layout version.
+ */
+int pyjac_hip_tables_load(const char *path, pyjac_mech_tables **out);
+int pyjac_hip_tables_free(pyjac_mech_tables *tables);
+
 /* Last error message of the calling thread ("" when none). */
 const char *pyjac_hip_last_error(void);
 
@@ -152,8 +166,9 @@ const char *pyjac_hip_last_error(void);
  *   conp        1 constant pressure, 0 constant volume (utils.py:972)
  *   order       'C' or 'F' (utils.py:944)
  *   jac_format  PYJAC_JAC_FULL or PYJAC_JAC_SPARSE (utils.py:1014)
- *   jac_type    PYJAC_JAC_EXACT, PYJAC_JAC_APPROXIMATE (unsupported) or
- *               PYJAC_JAC_FINITE_DIFFERENCE (utils.py:999)
+ *   jac_type    PYJAC_JAC_EXACT, PYJAC_JAC_APPROXIMATE (tables on the
+ *               approximate pattern) or PYJAC_JAC_FINITE_DIFFERENCE
+ *               (utils.py:999)
  */
 int pyjac_hip_create(const pyjac_mech_tables *mech, int conp, char order,
                      int jac_format, int jac_type, pyjac_handle **out);

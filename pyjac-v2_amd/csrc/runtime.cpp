@@ -16,6 +16,8 @@
 #include <mutex>
 #include <string>
 #include <vector>
+#include <type_traits>
+#include <cstddef>
 
 #include "../../include/pyjac_hip.h"
 #include "device_mech.h"
@@ -495,6 +497,94 @@ int ensure_staging(pyjac_handle* h) {
 extern "C" {
 
 const char* pyjac_hip_last_error(void) { return g_last_error.c_str(); }
+
+// ---------------------------------------------------------------- tables
+// the array fields of pyjac_mech_tables in struct order, with their types
+#define PJ_TABLE_ARRAYS(X)                                                               \
+    X(int32_t, rxn_off) X(int32_t, rxn_spec) X(int32_t, nu_prod) X(int32_t, nu_reac)    \
+    X(int32_t, nu_sum) X(int32_t, rev_mask) X(int32_t, thd_mask) X(int32_t, simple_mask) \
+    X(int32_t, plog_mask) X(int32_t, cheb_mask) X(int32_t, fall_mask)                    \
+    X(int32_t, simple_form) X(double, simple_A) X(double, simple_b) X(double, simple_Ta)  \
+    X(int32_t, fall_map) X(int32_t, fall_type) X(int32_t, fall_form)                     \
+    X(int32_t, fall_blend) X(int32_t, fall_param) X(int32_t, fall_to_thd)                \
+    X(double, fall_A) X(double, fall_b) X(double, fall_Ta) X(double, troe_a)             \
+    X(double, troe_T1inv) X(double, troe_T2) X(double, troe_T3inv) X(double, sri_a)      \
+    X(double, sri_b) X(double, sri_c) X(double, sri_d) X(double, sri_e)                  \
+    X(int32_t, thd_map) X(int32_t, thd_type) X(int32_t, thd_off) X(int32_t, thd_spec)    \
+    X(double, thd_eff) X(int32_t, plog_map) X(int32_t, plog_np) X(double, plog_params)   \
+    X(int32_t, cheb_map) X(int32_t, cheb_nt) X(int32_t, cheb_np) X(double, cheb_params)  \
+    X(double, cheb_plim) X(double, cheb_tlim) X(double, a_lo) X(double, a_hi)            \
+    X(double, T_mid) X(double, mw_post) X(int32_t, rxn_has_ns) X(int32_t, ccs_col_ptr)   \
+    X(int32_t, ccs_row_ind) X(int32_t, crs_row_ptr) X(int32_t, crs_col_ind)
+
+// the leading int32 fields, abi_version .. jac_nnz
+constexpr int PJ_TABLE_SCALARS = 18;
+
+struct LoadedTables {
+    pyjac_mech_tables t;
+    std::vector<std::vector<char>> blocks;
+};
+
+int pyjac_hip_tables_load(const char* path, pyjac_mech_tables** out) {
+    if (!path || !out) return fail(PYJAC_ERR_ARG, "null argument");
+    *out = nullptr;
+    FILE* fp = std::fopen(path, "rb");
+    if (!fp) return fail(PYJAC_ERR_IO, "could not open tables file: %s", path);
+    auto* L = new LoadedTables();
+    std::memset(&L->t, 0, sizeof(L->t));
+    int rc = PYJAC_OK;
+    auto rd = [&](void* p, size_t n) { return std::fread(p, 1, n, fp) == n; };
+    char magic[8];
+    int32_t ns = 0, na = 0;
+    int32_t sc[PJ_TABLE_SCALARS];
+    if (!rd(magic, 8) || !rd(&ns, 4)) rc = fail(PYJAC_ERR_IO, "tables file too short: %s", path);
+    else if (std::memcmp(magic, "PJHIPTB1", 8) != 0)
+        rc = fail(PYJAC_ERR_ARG, "not a pyJac HIP tables file: %s", path);
+    else if (ns != PJ_TABLE_SCALARS || !rd(sc, sizeof(sc)) || !rd(&na, 4))
+        rc = fail(PYJAC_ERR_ARG, "tables file %s: %d scalar fields, this library has %d", path, ns,
+                  PJ_TABLE_SCALARS);
+    if (!rc) {
+        int32_t* dst = &L->t.abi_version;  // the 18 leading int32 fields are contiguous
+        for (int i = 0; i < PJ_TABLE_SCALARS; ++i) dst[i] = sc[i];
+        if (L->t.abi_version != PYJAC_MECH_ABI_VERSION)
+            rc = fail(PYJAC_ERR_ARG, "tables file %s: ABI %d, library expects %d", path,
+                      L->t.abi_version, PYJAC_MECH_ABI_VERSION);
+    }
+    int idx = 0;
+#define PJ_LOAD(T, name)                                                                  \
+    if (!rc) {                                                                            \
+        uint8_t ty;                                                                       \
+        int64_t cnt;                                                                      \
+        if (idx >= na || !rd(&ty, 1) || !rd(&cnt, 8) || cnt < 0)                          \
+            rc = fail(PYJAC_ERR_IO, "tables file %s: array %d (" #name ") missing", path, idx); \
+        else if (ty != (std::is_same<T, double>::value ? 1 : 0))                          \
+            rc = fail(PYJAC_ERR_ARG, "tables file %s: array " #name " has another type", path); \
+        else {                                                                            \
+            L->blocks.emplace_back((size_t)std::max<int64_t>(cnt, 1) * sizeof(T));        \
+            if (!rd(L->blocks.back().data(), (size_t)cnt * sizeof(T)))                   \
+                rc = fail(PYJAC_ERR_IO, "tables file %s: array " #name " truncated", path); \
+            L->t.name = (const T*)L->blocks.back().data();                                \
+        }                                                                                 \
+        ++idx;                                                                            \
+    }
+    PJ_TABLE_ARRAYS(PJ_LOAD)
+#undef PJ_LOAD
+    if (!rc && idx != na)
+        rc = fail(PYJAC_ERR_ARG, "tables file %s: %d arrays, this library reads %d", path, na, idx);
+    std::fclose(fp);
+    if (rc) {
+        delete L;
+        return rc;
+    }
+    static_assert(offsetof(LoadedTables, t) == 0, "tables first");
+    *out = &L->t;
+    return PYJAC_OK;
+}
+
+int pyjac_hip_tables_free(pyjac_mech_tables* tables) {
+    delete reinterpret_cast<LoadedTables*>(tables);
+    return PYJAC_OK;
+}
 
 // read_initial_conditions.cpp.in:41-116: one record per state, C order on disk
 int pyjac_hip_read_initial_conditions(const char* filename, size_t num_states, int nn,

@@ -12,7 +12,12 @@ this writes into ``build_path``
   this mechanism and these options;
 * the mechanism (and thermo) file it was made from, and ``pyjac_hip.json``
   with the options;
-* ``libpyjac_hip.so``, copied from the package build (built on demand).
+* ``libpyjac_hip.so``, copied from the package build (built on demand);
+* ``tables.pjt`` (the mechanism tables for C and C++ callers),
+  ``pyjac_hip_main.cfg`` and the executable ``pyjac_hip_main``, the
+  counterpart of the reference's generated main (kernel.cpp.in:645-748):
+  ``pyjac_hip_main <build_path> <problem_size> <work_size>`` reads
+  ``data_filename``, runs the kernel and prints the timing line.
 """
 
 import json
@@ -36,7 +41,8 @@ import os as _os
 _here = _os.path.dirname(_os.path.abspath(__file__))
 with open(_os.path.join(_here, 'pyjac_hip.json')) as _f:
     _opts = _json.load(_f)
-_os.environ.setdefault('PYJAC_HIP_LIB', _os.path.join(_here, 'libpyjac_hip.so'))
+if _os.path.exists(_os.path.join(_here, 'libpyjac_hip.so')):
+    _os.environ.setdefault('PYJAC_HIP_LIB', _os.path.join(_here, 'libpyjac_hip.so'))
 
 from pyjac.core.mech_tables import MechTables as _MechTables  # noqa: E402
 from pyjac.pywrap import bind as _bind  # noqa: E402
@@ -58,6 +64,7 @@ def create_jacobian(lang, mech_name=None, therm_name=None, build_path='./out/',
                     rate_specialization=RateSpecialization.hybrid, conp=True,
                     jac_type=JacobianType.exact, jac_format=JacobianFormat.sparse,
                     fd_order=1, fd_mode=FiniteDifferenceMode.forward,
+                    data_filename='data.bin', for_validation=False,
                     build_library=True, **ignored):
     """Write the HIP kernel module for one mechanism; returns its path."""
     if lang != 'hip':
@@ -84,16 +91,28 @@ def create_jacobian(lang, mech_name=None, therm_name=None, build_path='./out/',
                 conp=bool(conp), order=data_order, kernel_type=KernelType(kernel_type).name)
     with open(os.path.join(build_path, 'pyjac_hip.json'), 'w') as f:
         json.dump(opts, f, indent=1)
+    # the executable's inputs (kernel.cpp.in:645-748): tables and options
+    mech.save_binary(os.path.join(build_path, 'tables.pjt'))
+    kt = KernelType(kernel_type)
+    with open(os.path.join(build_path, 'pyjac_hip_main.cfg'), 'w') as f:
+        f.write('tables tables.pjt\n')
+        f.write('kernel {}\n'.format('jacobian' if kt == KernelType.jacobian else 'species_rates'))
+        f.write('conp {}\norder {}\njac_format {}\njac_type {}\n'.format(
+            int(bool(conp)), data_order, opts['jac_format'], opts['jac_type']))
+        f.write('data {}\nfor_validation {}\n'.format(data_filename, int(bool(for_validation))))
     path = os.path.join(build_path, 'pyjac_hip.py')
     with open(path, 'w') as f:
         f.write(MODULE.format(mech=mech.name, conp=opts['conp'], order=data_order,
                               jac_format=opts['jac_format'], jac_type=opts['jac_type'],
                               rate_spec=opts['rate_spec'], kernel_type=opts['kernel_type']))
     if build_library:
-        from ..libgen import LIB_PATH, build_library as _build
+        from ..libgen import LIB_PATH, MAIN_NAME, build_library as _build
         if not os.path.exists(LIB_PATH):
             _build()
         shutil.copy2(LIB_PATH, os.path.join(build_path, os.path.basename(LIB_PATH)))
+        main = os.path.join(os.path.dirname(LIB_PATH), MAIN_NAME)
+        if os.path.exists(main):
+            shutil.copy2(main, os.path.join(build_path, MAIN_NAME))
     logging.getLogger(__name__).info('wrote %s (%d species, %d reactions)', path, mech.Ns,
                                      mech.Nr)
     return path

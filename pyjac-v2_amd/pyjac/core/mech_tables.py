@@ -225,6 +225,30 @@ class MechTables(object):
         """``ctypes`` pointer to the filled ``pyjac_mech_tables``."""
         return ctypes.pointer(self.cstruct)
 
+    TABLES_MAGIC = b'PJHIPTB1'
+
+    def save_binary(self, path):
+        """Write the tables as a binary file that C and C++ callers load with
+        ``pyjac_hip_tables_load`` (include/pyjac_hip.h): the magic
+        ``PJHIPTB1``, the number of scalar fields and their int32 values, the
+        number of arrays, then per array (in the struct's field order) its
+        element type (int32 0 / float64 1), int64 length and data."""
+        scalars = [(n, t) for n, t in _FIELDS if t is ctypes.c_int32]
+        arrays = [(n, t) for n, t in _FIELDS if t is not ctypes.c_int32]
+        with open(path, 'wb') as f:
+            f.write(self.TABLES_MAGIC)
+            f.write(np.int32(len(scalars)).tobytes())
+            f.write(np.array([getattr(self.cstruct, n) for n, _ in scalars], dtype=np.int32)
+                    .tobytes())
+            f.write(np.int32(len(arrays)).tobytes())
+            for n, t in arrays:
+                is_d = t is _D
+                a = np.ascontiguousarray(self.arrays[n], dtype=np.float64 if is_d else np.int32)
+                f.write(np.uint8(1 if is_d else 0).tobytes())
+                f.write(np.int64(a.size).tobytes())
+                f.write(a.tobytes())
+        return path
+
     # sparse index helpers --------------------------------------------------
     def sparse_rows_cols(self, order):
         """(rows, cols) of each stored sparse value: CCS for 'F', CRS for

@@ -50,6 +50,23 @@ def build_library(out_path=LIB_PATH, extra_flags=(), verbose=False):
     if verbose:
         print(res.stdout)
     os.replace(tmp, out_path)
+    build_main(os.path.join(os.path.dirname(out_path), MAIN_NAME), os.path.dirname(out_path))
+    return out_path
+
+
+MAIN_NAME = 'pyjac_hip_main'
+
+
+def build_main(out_path, lib_dir):
+    """Compile the executable counterpart of the reference's generated main
+    (csrc/pyjac_main.cpp, kernel_utils/common/kernel.cpp.in:645-748) against
+    libpyjac_hip.so in lib_dir (found next to the executable at run time)."""
+    cmd = [shutil.which('g++') or 'g++', '-O2', '-std=c++17', '-I', INCLUDE,
+           os.path.join(CSRC, 'pyjac_main.cpp'), '-L', lib_dir, '-lpyjac_hip',
+           '-Wl,-rpath,$ORIGIN:/opt/rocm/lib', '-o', out_path]
+    res = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if res.returncode != 0:
+        raise RuntimeError('g++ failed:\n' + res.stdout)
     return out_path
 
 

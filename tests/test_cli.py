@@ -69,3 +69,50 @@ def test_generated_module_runs(tmp_path):
     ref = oracle.jacobian(mod.mech, True, param, phi)
     fro = np.linalg.norm((jac - ref).reshape(n, -1), axis=1) / np.linalg.norm(ref.reshape(n, -1), axis=1)
     assert fro.max() <= 1e-8
+
+
+def test_build_folder_for_the_executable(tmp_path):
+    """The build folder holds what the executable needs (kernel.cpp.in:645-748):
+    the tables file, its configuration and the program."""
+    mech, therm = mechanism_files('h2o2')
+    create_jacobian('hip', mech, therm, build_path=str(tmp_path), data_order='F', conp=False,
+                    jac_format=1, for_validation=True)
+    cfg = dict(l.split() for l in (tmp_path / 'pyjac_hip_main.cfg').read_text().splitlines())
+    assert cfg == {'tables': 'tables.pjt', 'kernel': 'jacobian', 'conp': '0', 'order': 'F',
+                   'jac_format': '1', 'jac_type': '0', 'data': 'data.bin', 'for_validation': '1'}
+    assert (tmp_path / 'tables.pjt').read_bytes()[:8] == b'PJHIPTB1'
+    assert os.access(tmp_path / 'pyjac_hip_main', os.X_OK)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('kernel', ['jacobian', 'species_rates'])
+def test_executable_main(tmp_path, kernel):
+    """pyjac_hip_main <build_path> <problem_size> <work_size>: reads data.bin,
+    prints problem_size,compilation_time,setup_time,runtime and, for
+    validation, writes the output that the checker reproduces."""
+    import oracle
+    from pyjac.core.enum_types import KernelType
+    from pyjac.core.states import sample_states, write_data_bin
+    mech, therm = mechanism_files('gri30')
+    create_jacobian('hip', mech, therm, build_path=str(tmp_path), data_order='F', conp=False,
+                    jac_format=1, kernel_type=KernelType[kernel], for_validation=True)
+    from pyjac.core.mech_tables import MechTables
+    mt = MechTables.from_file(mech, therm)
+    n = 500
+    param, phi = sample_states(mt, n, seed=21, conp=False)
+    write_data_bin(str(tmp_path / 'data.bin'), param, phi)
+    out = subprocess.run([str(tmp_path / 'pyjac_hip_main'), str(tmp_path), str(n), '128'],
+                         capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    fields = out.stdout.strip().split(',')
+    assert int(fields[0]) == n and float(fields[1]) == -1.0 and float(fields[3]) > 0
+    if kernel == 'jacobian':
+        got = np.fromfile(tmp_path / 'jac.bin').reshape(-1, n).T
+        ref = oracle.to_layout(mt, oracle.jacobian(mt, False, param, phi), 'F', 1).reshape(-1, n).T
+        fro = np.linalg.norm(got - ref, axis=1) / np.linalg.norm(ref, axis=1)
+        assert fro.max() <= 1e-8
+    else:
+        got = np.fromfile(tmp_path / 'dphi.bin').reshape(-1, n).T
+        from rate_tol import rate_error, rate_scales
+        ref, scale = rate_scales(mt, False, param, phi)
+        assert rate_error(got, ref, scale) <= 1.0

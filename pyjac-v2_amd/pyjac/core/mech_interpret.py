@@ -508,8 +508,16 @@ def reassign_species_lists(reacs, specs):
 def load_mechanism(mech_filename, therm_filename=None, last_spec=None,
                    sort_type=None):
     """Parse, move the bath gas last and index the species lists; the same
-    front-end sequence as reference create_jacobian.py:5458-5470."""
-    elems, specs, reacs = read_mech(mech_filename, therm_filename, sort_type)
+    front-end sequence as reference create_jacobian.py:5434-5470: Cantera
+    ``.cti`` files go through the CTI reader, anything else is Chemkin."""
+    name = str(mech_filename)
+    if name.endswith(('.xml', '.yaml', '.yml')):
+        raise MechanismError('Unsupported mechanism format (Chemkin or Cantera CTI): ' + name)
+    if name.endswith('.cti'):
+        from .cti import read_mech_ct
+        elems, specs, reacs = read_mech_ct(name, sort_type)
+    else:
+        elems, specs, reacs = read_mech(mech_filename, therm_filename, sort_type)
     if not specs:
         raise MechanismError('No species found in ' + mech_filename)
     if not reacs:

@@ -1,0 +1,217 @@
+"""Cantera CTI input (reference core/mech_interpret.py:972, dispatched at
+core/create_jacobian.py:5434): the CTI reader must produce the same tables
+as the Chemkin reader for the same mechanism.
+
+Pinned by: the reference's own pair of test mechanisms (pyjac/tests/test.cti,
+its Chemkin twin test.inp: every reaction type, bit-identical tables), the
+reference's data/h2o2.cti against data/h2o2.inp (same chemistry, plus an N2
+species: equal rates on states without N2), and one mechanism written twice,
+in Chemkin units and in Cantera's SI defaults, whose SI numbers were
+converted by hand (1 cm^3/mol = 1e-3 m^3/kmol per collider order, 1 cal/mol =
+4184 J/kmol, 1 atm = 101325 Pa, log10 k shifts by -3 per order)."""
+
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from pyjac.core.chem_model import PA, RU
+from pyjac.core.cti import cti_to_chemkin, parse_cti
+from pyjac.core.exceptions import MechanismError
+from pyjac.core.mech_tables import MechTables
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+MECH = os.path.join(ROOT, 'tests', 'golden', 'mechanisms')
+
+
+def _compare_arrays(a, b, rtol=0.0):
+    assert a.species_names == b.species_names
+    assert a.reaction_strings == b.reaction_strings
+    assert sorted(a.arrays) == sorted(b.arrays)
+    for k in a.arrays:
+        x, y = np.asarray(a.arrays[k]), np.asarray(b.arrays[k])
+        assert x.shape == y.shape, k
+        if x.dtype.kind == 'f' and rtol:
+            np.testing.assert_allclose(y, x, rtol=rtol, atol=0, err_msg=k)
+        else:
+            assert np.array_equal(x, y), k
+
+
+def test_reference_test_mechanism_tables():
+    """test.cti and test.inp hold the same 40 species and 31 reactions
+    (elementary, three-body, Lindemann/Troe/SRI falloff, chemically activated,
+    PLOG, Chebyshev, duplicates, irreversible, named bath gases)."""
+    a = MechTables.from_file(os.path.join(MECH, 'test.inp'))
+    b = MechTables.from_file(os.path.join(MECH, 'test.cti'))
+    assert (b.Ns, b.Nr) == (40, 31)
+    _compare_arrays(a, b)
+
+
+@pytest.mark.parametrize('conp', [True, False])
+def test_h2o2_rates_match_chemkin(conp):
+    """data/h2o2.cti adds N2 (the last species) to data/h2o2.inp (Ar last):
+    on states without N2 both give the same rates."""
+    ck = MechTables.from_file(os.path.join(MECH, 'h2o2.inp'))
+    ct = MechTables.from_file(os.path.join(MECH, 'h2o2.cti'))
+    assert ck.reaction_strings == ct.reaction_strings
+    assert ct.species_names == ck.species_names[:-1] + ['AR', 'N2']
+    for i, s in enumerate(ck.species_names):
+        j = ct.species_names.index(s)
+        for key in ('a_lo', 'a_hi'):
+            np.testing.assert_array_equal(ck.arrays[key].reshape(-1, 7)[i],
+                                          ct.arrays[key].reshape(-1, 7)[j])
+    rng = np.random.default_rng(3)
+    n = 64
+    T = rng.uniform(900.0, 2400.0, n)
+    P = rng.uniform(1.0, 30.0, n) * PA
+    V = np.ones(n)
+    ntot = P * V / (RU * T)
+    X = rng.dirichlet(np.full(ck.Ns, 0.5), n)
+    moles = X * ntot[:, None]
+    phi_ck = np.column_stack([T, V if conp else P, moles[:, :-1]])
+    # the same state with N2 (implicit, last) at zero: Ar becomes explicit
+    phi_ct = np.column_stack([T, V if conp else P, moles])
+    param = P if conp else V
+    r_ck = oracle.species_rates(ck, conp, param, phi_ck)
+    r_ct = oracle.species_rates(ct, conp, param, phi_ct)
+    # N2 = ntot - sum(n) is a rounding residue, not zero
+    scale = np.abs(r_ck).max(axis=1, keepdims=True)
+    assert (np.abs(r_ct[:, :ck.NN] - r_ck) <= 1e-12 * np.abs(r_ck) + 1e-14 * scale).all()
+
+
+_PHASE = '''
+ideal_gas(name='gas', elements="O H Ar N", species="""H2 H O O2 OH H2O HO2 H2O2 AR N2""",
+          reactions='all', initial_state=state(temperature=300.0, pressure=OneAtm))
+'''
+
+_CGS = '''units(length='cm', time='s', quantity='mol', act_energy='cal/mol')
+reaction('O + H2 <=> H + OH', [3.87e4, 2.7, 6260.0])
+three_body_reaction('O + H + M <=> OH + M', [5.0e17, -1.0, 0.0], efficiencies='AR:0.7 H2:2.0')
+falloff_reaction('2 OH (+ M) <=> H2O2 (+ M)', kf=[7.4e13, -0.37, 0.0],
+                 kf0=[2.3e18, -0.9, -1700.0], efficiencies='AR:0.7 H2:2.0 H2O:6.0',
+                 falloff=Troe(A=0.7346, T3=94.0, T1=1756.0, T2=5182.0))
+chemically_activated_reaction('H + O2 (+ M) <=> OH + O (+ M)', kLow=[4.989e12, 0.099, 10600.0],
+                              kHigh=[3.802e-7, 4.838, 7710.0],
+                              falloff=SRI(A=1.641, B=4334.0, C=2725.0))
+pdep_arrhenius('H + HO2 <=> H2 + O2', [(0.04, 'atm'), 3.398e13, -1.9, 9279.0],
+               [(1.0, 'atm'), 9.362e12, -1.28, 14240.0])
+chebyshev_reaction('HO2 + OH (+ M) <=> H2O + O2 (+ M)', Tmin=290.0, Tmax=3000.0,
+                   Pmin=(0.001, 'atm'), Pmax=(100.0, 'atm'),
+                   coeffs=[[8.0, 0.1], [0.5, -0.02], [0.1, 0.01]])
+reaction('H2O2 + H => HO2 + H2', Arrhenius(A=1.21e7, b=2.0, E=5200.0), options='duplicate')
+reaction('H2O2 + H => HO2 + H2', [1.0e3, 0.0, 0.0], options=['duplicate'])
+'''
+
+# the same mechanism in Cantera's default units (m, kmol, J/kmol, Pa),
+# converted by hand
+_SI = '''
+reaction('O + H2 <=> H + OH', [38.7, 2.7, 26191840.0])
+three_body_reaction('O + H + M <=> OH + M', [5.0e11, -1.0, 0.0], efficiencies='AR:0.7 H2:2.0')
+falloff_reaction('2 OH (+ M) <=> H2O2 (+ M)', kf=[7.4e10, -0.37, 0.0],
+                 kf0=[2.3e12, -0.9, -7112800.0], efficiencies='AR:0.7 H2:2.0 H2O:6.0',
+                 falloff=Troe(A=0.7346, T3=94.0, T1=1756.0, T2=5182.0))
+chemically_activated_reaction('H + O2 (+ M) <=> OH + O (+ M)', kLow=[4.989e9, 0.099, 44350400.0],
+                              kHigh=[3.802e-7, 4.838, 32258640.0],
+                              falloff=SRI(A=1.641, B=4334.0, C=2725.0))
+pdep_arrhenius('H + HO2 <=> H2 + O2', [4053.0, 3.398e10, -1.9, 38823336.0],
+               [101325.0, 9.362e9, -1.28, 59580160.0])
+chebyshev_reaction('HO2 + OH (+ M) <=> H2O + O2 (+ M)', Tmin=290.0, Tmax=3000.0,
+                   Pmin=101.325, Pmax=(100.0, 'atm'),
+                   coeffs=[[5.0, 0.1], [0.5, -0.02], [0.1, 0.01]])
+reaction('H2O2 + H => HO2 + H2', Arrhenius(A=1.21e4, b=2.0, E=21756800.0), options='duplicate')
+reaction('H2O2 + H => HO2 + H2', [1.0, 0.0, 0.0], options=['duplicate'])
+'''
+
+
+def _species_block():
+    text = open(os.path.join(MECH, 'h2o2.cti')).read()
+    start = text.index('species(name')
+    end = text.index('#', text.index('# Reaction data') - 90)
+    return text[start:end]
+
+
+def _tables(tmp_path, name, body):
+    path = tmp_path / (name + '.cti')
+    path.write_text(body)
+    return MechTables.from_file(str(path))
+
+
+def test_units_conversion(tmp_path):
+    species = _species_block()
+    cgs = _tables(tmp_path, 'cgs', _CGS.split('\n', 1)[0] + _PHASE + species
+                  + _CGS.split('\n', 1)[1])
+    si = _tables(tmp_path, 'si', _PHASE + species + _SI)
+    assert cgs.Nr == 8
+    _compare_arrays(cgs, si, rtol=1e-14)
+
+
+def test_zero_rate_reactions_are_dropped(tmp_path):
+    """mech_interpret.py:1209-1211: an elementary reaction with A = 0 is
+    skipped."""
+    body = _PHASE + _species_block() + "reaction('O + H2 <=> H + OH', [3.87e4, 2.7, 6260.0])\n" \
+        "reaction('H + O2 <=> O + OH', [0.0, 0.0, 0.0])\n"
+    assert _tables(tmp_path, 'zero', body).Nr == 1
+
+
+def test_named_collider_and_default_efficiency(tmp_path):
+    """A falloff with a zero default efficiency and a single efficiency of one
+    is the "(+SP)" form (mech_interpret.py:1066-1073); a three-body default
+    efficiency other than one applies to every unlisted species
+    (:1074-1083)."""
+    species = _species_block()
+    named = _tables(tmp_path, 'named', _PHASE + species + (
+        "falloff_reaction('2 OH (+ M) <=> H2O2 (+ M)', kf=[7.4e13, -0.37, 0.0], "
+        "kf0=[2.3e18, -0.9, -1700.0], efficiencies='H2O:1.0', default_efficiency=0.0)\n"))
+    paren = _tables(tmp_path, 'paren', _PHASE + species + (
+        "falloff_reaction('2 OH (+ H2O) <=> H2O2 (+ H2O)', kf=[7.4e13, -0.37, 0.0], "
+        "kf0=[2.3e18, -0.9, -1700.0])\n"))
+    _compare_arrays(paren, named)
+    dflt = _tables(tmp_path, 'dflt', _PHASE + species + (
+        "three_body_reaction('O + H + M <=> OH + M', [5.0e17, -1.0, 0.0], "
+        "efficiencies='H2:2.0', default_efficiency=0.5)\n"))
+    listed = _tables(tmp_path, 'listed', _PHASE + species + (
+        "three_body_reaction('O + H + M <=> OH + M', [5.0e17, -1.0, 0.0], efficiencies="
+        "'H2:2.0 H:0.5 O:0.5 O2:0.5 OH:0.5 H2O:0.5 HO2:0.5 H2O2:0.5 AR:0.5 N2:0.5')\n"))
+    _compare_arrays(listed, dflt)
+
+
+@pytest.mark.parametrize('text', [
+    "import os\n",
+    "x = 1\n",
+    "__import__('os').system('true')\n",
+    "reaction('O + H2 <=> H + OH', [open('f').read(), 2.7, 6260.0])\n",
+    "reaction('O + H2 <=> H + OH', [a, 2.7, 6260.0])\n",
+    "reaction('O + H2 <=> H + OH', [lambda: 1, 2.7, 6260.0])\n",
+])
+def test_nothing_is_executed(text):
+    """The file is read as a syntax tree; statements, attribute calls, names
+    and lambdas are refused rather than run."""
+    with pytest.raises(MechanismError):
+        cti_to_chemkin(_PHASE + text)
+
+
+def test_parse_records():
+    calls = parse_cti("units(length='cm', quantity='mol')\nNASA([200.0, 1000.0], [1, -2e3])\n")
+    assert calls[0].name == 'units' and calls[0].kwargs == {'length': 'cm', 'quantity': 'mol'}
+    assert calls[1].args == [[200.0, 1000.0], [1, -2000.0]]
+
+
+@pytest.mark.parametrize('body,msg', [
+    ("units(length='furlong')\n", 'unsupported unit'),
+    ("surface_reaction('O + H2 <=> H + OH', [1, 0, 0])\n", 'unsupported reaction'),
+    ("reaction('O + H2 <=> H + OH', [(1.0, 'cm3/mol/s'), 0, 0])\n", 'explicit units'),
+])
+def test_unsupported_input_is_an_error(body, msg):
+    with pytest.raises(MechanismError, match=msg):
+        cti_to_chemkin(_PHASE + _species_block() + body)
+
+
+def test_command_line_accepts_cti(tmp_path):
+    out = subprocess.run([sys.executable, '-m', 'pyjac', '-l', 'hip', '-i',
+                          os.path.join(MECH, 'h2o2.cti'), '-b', str(tmp_path), '-o', 'C'],
+                         cwd=os.path.join(ROOT, 'pyjac-v2_amd'), capture_output=True, text=True)
+    assert out.returncode == 0, out.stderr
+    assert (tmp_path / 'pyjac_hip.py').exists()

@@ -1367,8 +1367,25 @@ __device__ __forceinline__ void jacobian_global_state(const BlockArgs* __restric
     }
 }
 
+// Occupancy targets (waves per SIMD) that the register allocator is held to.
+// Unconstrained, the Jacobian kernels take 171-181 VGPRs (2 waves per SIMD)
+// although a GRI-sized workgroup's LDS (30 KB) leaves room for 2.5; held to
+// 3 waves (168 VGPRs, no spills) GRI runs 12 % faster, H2/O2 25 % and the
+// device-memory variant on 150 species 15 %.  The species-rate kernel at 5
+// waves (96 VGPRs and a 12-register spill, from 111 VGPRs and 4 waves) gains
+// 11 %.  Four waves for the Jacobian kernels spill 48-56 registers and lose.
+#ifndef PJB_JAC_WAVES
+#define PJB_JAC_WAVES 3
+#endif
+#ifndef PJB_RATES_WAVES
+#define PJB_RATES_WAVES 5
+#endif
+#ifndef PJB_GLOB_WAVES
+#define PJB_GLOB_WAVES 3
+#endif
+
 template <int WPS>
-__global__ void __launch_bounds__(64 * WPS)
+__global__ void __launch_bounds__(64 * WPS) __attribute__((amdgpu_waves_per_eu(PJB_RATES_WAVES)))
 rates_block_kernel(const BlockArgs* __restrict__ A, int conp, long n,
                    const double* __restrict__ param, const double* __restrict__ phi,
                    double* __restrict__ dphi, Layout L) {
@@ -1379,7 +1396,7 @@ rates_block_kernel(const BlockArgs* __restrict__ A, int conp, long n,
 }
 
 template <int WPS>
-__global__ void __launch_bounds__(64 * WPS)
+__global__ void __launch_bounds__(64 * WPS) __attribute__((amdgpu_waves_per_eu(PJB_JAC_WAVES)))
 jacobian_block_kernel(const BlockArgs* __restrict__ A, int conp, long n,
                       const double* __restrict__ param, const double* __restrict__ phi,
                       double* __restrict__ jac, Layout L, int out_C) {
@@ -1393,7 +1410,7 @@ jacobian_block_kernel(const BlockArgs* __restrict__ A, int conp, long n,
 // contributions at cbuf + i * ncontrib and its entries at out + i * nout (the
 // output itself in C order, a chunk buffer in F order)
 template <int WPS>
-__global__ void __launch_bounds__(64 * WPS)
+__global__ void __launch_bounds__(64 * WPS) __attribute__((amdgpu_waves_per_eu(PJB_GLOB_WAVES)))
 jacobian_global_kernel(const BlockArgs* __restrict__ A, int conp, long s0, long count,
                        const double* __restrict__ param, const double* __restrict__ phi,
                        double* __restrict__ cbuf, double* __restrict__ out, Layout L) {
@@ -1484,7 +1501,8 @@ __device__ __forceinline__ int ragged_member(const RaggedArgs& R, long b) {
 }
 
 template <int WPS>
-__global__ void __launch_bounds__(64 * WPS) rates_ragged_kernel(RaggedArgs R) {
+__global__ void __launch_bounds__(64 * WPS) __attribute__((amdgpu_waves_per_eu(PJB_RATES_WAVES)))
+rates_ragged_kernel(RaggedArgs R) {
     extern __shared__ double smem[];
     const int i = ragged_member(R, blockIdx.x);
     const RaggedMember& M = R.m[i];
@@ -1494,7 +1512,8 @@ __global__ void __launch_bounds__(64 * WPS) rates_ragged_kernel(RaggedArgs R) {
 }
 
 template <int WPS>
-__global__ void __launch_bounds__(64 * WPS) jacobian_ragged_kernel(RaggedArgs R) {
+__global__ void __launch_bounds__(64 * WPS) __attribute__((amdgpu_waves_per_eu(PJB_JAC_WAVES)))
+jacobian_ragged_kernel(RaggedArgs R) {
     extern __shared__ double smem[];
     const int i = ragged_member(R, blockIdx.x);
     const RaggedMember& M = R.m[i];

@@ -50,7 +50,8 @@ def build_library(out_path=LIB_PATH, extra_flags=(), verbose=False):
     if verbose:
         print(res.stdout)
     os.replace(tmp, out_path)
-    build_main(os.path.join(os.path.dirname(out_path), MAIN_NAME), os.path.dirname(out_path))
+    if os.path.basename(out_path) == LIB_NAME:
+        build_main(os.path.join(os.path.dirname(out_path), MAIN_NAME), os.path.dirname(out_path))
     return out_path
 
 

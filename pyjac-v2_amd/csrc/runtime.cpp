@@ -161,9 +161,12 @@ int block_wps(const pyjac_handle* h, int kernel) {
     int v = h->variant[kernel];
     if (v > 10) return v - 10;
     if (h->wps[kernel]) return h->wps[kernel];
-    // measured on MI355X (GRI, 1M states): rates fastest with one wave per
-    // state, the Jacobian with two
-    return kernel == 1 ? 2 : 1;
+    // measured on MI355X: rates fastest with one wave per state; the
+    // Jacobian with two for GRI (325 reactions: 2.5e7/s against 1.9e7/s with
+    // one) but one for H2/O2 (27 reactions: 1.4e8/s against 8.2e7/s), whose
+    // reactions fill less than one wave's pass
+    if (kernel == 1) return h->m.Nr >= 128 ? 2 : 1;
+    return 1;
 }
 
 bool use_block(const pyjac_handle* h, int kernel) {

@@ -1,0 +1,215 @@
+"""Validation harness (reference pyjac/functional_tester): the PaSR state
+generator and the error norms, on the CPU with the checker's integrator.
+
+Pinned by: the reference's own PaSR output (data/h2_pasr_output.npy, kept
+as tests/golden/h2_pasr_states.npz), whose particles start at the pilot's
+adiabatic equilibrium computed by Cantera -- our equilibrium of the same
+mixture must match it; conservation laws a premixed PaSR cannot break (every
+particle keeps the fresh mixture's enthalpy and element mass fractions);
+textbook values (stoichiometric mixtures, the enthalpy of formation of
+water); hand-computed norms on small arrays."""
+
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from pyjac.core.mech_tables import MechTables
+from pyjac.functional_tester import pasr
+from pyjac.functional_tester.norms import INF_CUTOFF, JacobianErrors, RateErrors, \
+    jacobian_threshold
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+MECH = os.path.join(ROOT, 'tests', 'golden', 'mechanisms')
+PASR = os.path.join(ROOT, 'tests', 'golden', 'h2_pasr_states.npz')
+AIR = {'O2': 1.0, 'N2': 3.76}
+
+
+@pytest.fixture(scope='module')
+def h2():
+    return MechTables.from_file(os.path.join(MECH, 'h2o2.cti'))
+
+
+def oracle_reactor(mech, rtol=1e-6, atol=1e-14):
+    def integrate(param, phi, dt):
+        y, st = oracle.integrate_ros2(mech, True, param, phi, dt, rtol=rtol, atol=atol)
+        assert st[3] == 0, st
+        return y
+    return pasr.phi_reactor(mech, integrate)
+
+
+def test_equivalence_ratio(h2):
+    th = pasr.MixtureThermo(h2)
+    X = pasr.equivalence_ratio(th, 1.0, {'H2': 1.0}, AIR, ['H2O', 'N2'])
+    # 2 H2 + (O2 + 3.76 N2)
+    assert X['H2'] == pytest.approx(2 / 6.76, rel=1e-14)
+    assert X['O2'] == pytest.approx(1 / 6.76, rel=1e-14)
+    assert X['N2'] == pytest.approx(3.76 / 6.76, rel=1e-14)
+    X = pasr.equivalence_ratio(th, 0.5, {'H2': 1.0}, AIR, ['H2O', 'N2'])
+    assert X['H2'] == pytest.approx(1 / 5.76, rel=1e-14)
+    gri = MechTables.from_file(*[os.path.join(MECH, f) for f in ('grimech30.inp', 'thermo30.dat')])
+    X = pasr.equivalence_ratio(pasr.MixtureThermo(gri), 1.0, {'CH4': 1.0}, AIR,
+                               ['CO2', 'H2O', 'N2'])
+    # CH4 + 2 (O2 + 3.76 N2)
+    assert X['CH4'] == pytest.approx(1 / 10.52, rel=1e-14)
+    with pytest.raises(ValueError):
+        pasr.equivalence_ratio(pasr.MixtureThermo(gri), 1.0, {'CH4': 1.0}, AIR, ['H2O', 'N2'])
+
+
+def test_thermo(h2):
+    th = pasr.MixtureThermo(h2)
+    names = h2.species_names
+    h = th.h_molar([298.15])[0]
+    # enthalpy of formation of water vapour, -241.826 kJ/mol (JANAF)
+    assert h[names.index('H2O')] == pytest.approx(-241.826e6, rel=2e-4)
+    # elements in their reference state: zero within the fits (10 J/mol)
+    assert abs(h[names.index("N2")]) < 1e4 and abs(h[names.index("O2")]) < 1e4
+    # heat capacity of N2 at 298.15 K, 29.12 J/mol/K (JANAF; the fit is within 0.2 %)
+    assert th.cp_molar([298.15])[0][names.index('N2')] == pytest.approx(29.12e3, rel=3e-3)
+    rng = np.random.default_rng(0)
+    Y = rng.dirichlet(np.ones(th.Ns), 20)
+    T = rng.uniform(300.0, 3000.0, 20)
+    np.testing.assert_allclose(th.temperature(th.h_mass(T, Y), Y), T, rtol=1e-12)
+
+
+def test_pilot_matches_the_reference_equilibrium(h2):
+    """The reference's particles all start at the pilot, the HP equilibrium
+    of stoichiometric H2/air at 300 K and 1 atm (Cantera)."""
+    ref = np.load(PASR)
+    assert list(ref['species']) == h2.species_names
+    th = pasr.MixtureThermo(h2)
+    X = pasr.equivalence_ratio(th, 1.0, {'H2': 1.0}, AIR, ['H2O', 'N2'])
+    Y0 = th.mole_to_mass(th.composition(X))[0]
+    T, Y = pasr.equilibrate(th, oracle_reactor(h2), 300.0, 101325.0, Y0)
+    assert abs(T - ref['T'][0]) < 1e-3
+    big = ref['Y'][0] > 1e-10
+    np.testing.assert_allclose(Y[big], ref['Y'][0][big], rtol=1e-5)
+
+
+@pytest.fixture(scope='module')
+def short_run(h2):
+    ref = np.load(PASR)
+    pilot = (ref['T'][0], ref['Y'][0])
+    kw = dict(complete_products=['H2O', 'N2'], num_part=10, tau_res=10e-3, tau_mix=1e-3,
+              tau_pair=1e-3, num_res=0.2, pilot=pilot)
+    data = pasr.run_simulation(h2, 'premixed', 300.0, 1.0, 1.0, {'H2': 1.0}, AIR, seed=3,
+                               react=oracle_reactor(h2), **kw)
+    again = pasr.run_simulation(h2, 'premixed', 300.0, 1.0, 1.0, {'H2': 1.0}, AIR, seed=3,
+                                react=oracle_reactor(h2), **kw)
+    return data, again
+
+
+def test_pasr_shape_and_reproducibility(short_run):
+    data, again = short_run
+    assert data.shape == (21, 10, 13)
+    np.testing.assert_allclose(data[:, 0, 0], np.arange(21) * 1e-4, rtol=1e-12, atol=1e-15)
+    assert np.array_equal(data, again)
+    np.testing.assert_allclose(data[..., 3:].sum(axis=-1), 1.0, rtol=1e-12)
+    assert np.all(data[..., 2] == 101325.0)
+    # fresh gas has entered and mixed
+    assert data[..., 1].min() < 1500.0
+
+
+def test_pasr_conserves_enthalpy_and_elements(h2, short_run):
+    """Premixed: the fresh stream and the pilot share the enthalpy and the
+    element composition; mixing and adiabatic constant-pressure reaction keep
+    both."""
+    data, _ = short_run
+    th = pasr.MixtureThermo(h2)
+    X = pasr.equivalence_ratio(th, 1.0, {'H2': 1.0}, AIR, ['H2O', 'N2'])
+    Y0 = th.mole_to_mass(th.composition(X))
+    h0 = th.h_mass([300.0], Y0)[0]
+    flat = data.reshape(-1, data.shape[-1])
+    T, Y = flat[:, 1], flat[:, 3:]
+    dh = th.h_mass(T, Y) - h0
+    assert np.abs(dh / th.cp_mass(T, Y)).max() < 1e-3  # kelvin
+    E = np.zeros((th.Ns, 4))
+    for k, sp in enumerate(h2.specs):
+        for e, n in sp.elem:
+            E[k, ['h', 'o', 'n', 'ar'].index(e.lower())] += n
+    z = (Y / th.W) @ E
+    z0 = (Y0 / th.W) @ E
+    # the bath gas follows from the ideal-gas law in the state vector, so it
+    # carries the integrator's error (rtol 1e-6) at a small fraction
+    np.testing.assert_allclose(z, np.broadcast_to(z0, z.shape), rtol=1e-7, atol=1e-15)
+    assert T.max() <= 2387.637 + 1e-3 and T.min() >= 300.0
+
+
+def test_particle_states(h2, short_run):
+    data, _ = short_run
+    param, phi = pasr.particle_states(h2, data, conp=True)
+    assert phi.shape[1] == h2.NN and param.shape[0] == phi.shape[0]
+    dphi = oracle.species_rates(h2, True, param, phi)
+    assert np.all(np.isfinite(dphi))
+
+
+def test_parse_input_file(tmp_path):
+    p = tmp_path / 'pasr.yaml'
+    p.write_text('case: premixed\ntemperature: 300.0\npressure: 1.0\nequivalence ratio: 1.0\n'
+                 'fuel:\n  H2: 1.0\noxidizer:\n  O2: 1.0\n  N2: 3.76\n'
+                 'complete products:\n  - H2O\n  - N2\nnumber of particles: 10\n')
+    pars = pasr.parse_input_file(str(p))
+    assert pars['number of particles'] == 10 and pars['number of residence times'] == 5
+    assert pars['mixing time'] == 1e-3
+    p.write_text('case: stirred\n')
+    with pytest.raises(ValueError):
+        pasr.parse_input_file(str(p))
+
+
+def test_jacobian_norms_by_hand():
+    out = np.array([1.0, 2.0, 0.0, 4.0])
+    ans = np.array([1.0, 2.5, 0.0, 5.0])
+    thr = jacobian_threshold([ans])
+    assert thr == pytest.approx(np.sqrt(1 + 6.25 + 25))
+    e = JacobianErrors(rtol=1e-6, atol=1e-2)
+    d = dict(e.update(out, ans, thr))
+    assert d['jac'] == pytest.approx(np.sqrt(0.2 ** 2 + 0.2 ** 2))
+    assert d['jac_zero'] == 0.0
+    assert d['jac_lapack'] == pytest.approx(np.sqrt(1.25) / np.sqrt(32.25))
+    assert d['jac_thresholded_20'] == pytest.approx(np.sqrt(0.08))
+    w = np.array([0.0, 0.5 / (0.01 + 2.5e-6), 0.0, 1.0 / (0.01 + 5e-6)])
+    assert d['jac_weighted_0.01'] == pytest.approx(np.linalg.norm(w))
+    assert d['jac_weighted_0.01_PJ_amax'] == 4.0
+    assert d['jac_weighted_0.01_AD_amax'] == pytest.approx(5.0)
+    assert d['jac_max_value'] == 4.0 and d['jac_threshold_value'] == thr
+    # a second chunk combines as sqrt(a^2 + b^2), maxima as maxima
+    d2 = e.update(out, ans, thr)
+    assert d2['jac_lapack'] == pytest.approx(np.sqrt(2) * d['jac_lapack'])
+    assert d2['jac_max_value'] == 4.0
+
+
+def test_jacobian_norms_infinities():
+    e = JacobianErrors()
+    out = np.array([1.0, 2e300])
+    e.update(out, np.array([1.0, np.inf]), 1.0)
+    assert e.errors['jac_lapack'] == 0.0
+    with pytest.raises(AssertionError):
+        JacobianErrors().update(out, np.array([1.0, 5.0]), 1.0)
+    # finite output against an infinite reference is not filtered: the norms
+    # say so (test.py checks the reference only where the output is infinite)
+    with np.errstate(invalid='ignore'):
+        d = JacobianErrors().update(np.array([1.0, 2.0]), np.array([1.0, np.inf]), 1.0)
+    assert not np.isfinite(d['jac_lapack'])
+    with pytest.raises(AssertionError):
+        JacobianErrors().update(np.array([np.nan, 2.0]), np.array([1.0, 2.0]), 1.0)
+    assert jacobian_threshold([np.array([3.0, np.inf, 2 * INF_CUTOFF, 4.0])]) == 5.0
+
+
+def test_rate_norms_by_hand():
+    e = RateErrors(rtol=1e-6, atol=1e-10)
+    ans = np.array([[1.0, 10.0], [2.0, 20.0], [3.0, 30.0]])
+    out = ans.copy()
+    out[1, 0] += 1e-5  # weighted 1e-5 / (1e-10 + 2e-6)
+    out[2, 1] -= 3e-5
+    d = e.update('dphi', out, ans)
+    np.testing.assert_allclose(d['dphi'], [1e-5, 3e-5], rtol=1e-9)
+    np.testing.assert_allclose(d['dphi_store'], [1e-5 / (1e-10 + 2e-6), 3e-5 / (1e-10 + 3e-5)],
+                               rtol=1e-9)
+    np.testing.assert_array_equal(d['dphi_value'], [2.0, 30.0])
+    # a later chunk replaces a column's record only where it is at least as bad
+    out2 = ans.copy()
+    out2[0, 0] += 1e-4
+    d = e.update('dphi', out2, ans)
+    np.testing.assert_allclose(d['dphi'], [1e-4, 3e-5], rtol=1e-9)
+    np.testing.assert_array_equal(d['dphi_value'], [1.0, 30.0])

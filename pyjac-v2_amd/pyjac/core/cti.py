@@ -334,11 +334,17 @@ _REACTIONS = ('reaction', 'three_body_reaction', 'falloff_reaction',
 def _convert(text):
     """(Chemkin text, {species: (lo, hi) NASA coefficients}) of the first
     phase of a CTI file."""
+    return convert_directives(parse_cti(text))
+
+
+def convert_directives(calls):
+    """(Chemkin text, {species: (lo, hi) NASA coefficients}) of a
+    mechanism given as CTI directives (Call records, in file order)."""
     units = _Units()
     phase = None
     species = {}
     reactions = []
-    for c in parse_cti(text):
+    for c in calls:
         if c.name == 'units':
             units = _Units(**c.kwargs)
         elif c.name == 'ideal_gas':
@@ -389,9 +395,14 @@ def read_mech_ct(filename, sort_type=None):
     them for the same mechanism (mech_interpret.py:972 returns the same
     triple from Cantera).  The NASA coefficients are set from the file's
     values themselves, not their fixed-width Chemkin rendering."""
-    from .mech_interpret import read_mech
     with open(filename) as f:
-        text, coeffs = _convert(f.read())
+        return read_converted(*_convert(f.read()), sort_type=sort_type)
+
+
+def read_converted(text, coeffs, sort_type=None):
+    """``(elems, specs, reacs)`` from converted Chemkin text and the exact
+    NASA coefficients."""
+    from .mech_interpret import read_mech
     with tempfile.TemporaryDirectory(prefix='pyjac_cti_') as d:
         path = os.path.join(d, 'mech.inp')
         with open(path, 'w') as f:

@@ -509,13 +509,18 @@ def load_mechanism(mech_filename, therm_filename=None, last_spec=None,
                    sort_type=None):
     """Parse, move the bath gas last and index the species lists; the same
     front-end sequence as reference create_jacobian.py:5434-5470: Cantera
-    ``.cti`` files go through the CTI reader, anything else is Chemkin."""
+    ``.cti`` and ``.yaml`` files go through their readers, anything else is
+    Chemkin."""
     name = str(mech_filename)
-    if name.endswith(('.xml', '.yaml', '.yml')):
-        raise MechanismError('Unsupported mechanism format (Chemkin or Cantera CTI): ' + name)
+    if name.endswith('.xml'):
+        raise MechanismError('Unsupported mechanism format (Chemkin, Cantera CTI or YAML): '
+                             + name)
     if name.endswith('.cti'):
         from .cti import read_mech_ct
         elems, specs, reacs = read_mech_ct(name, sort_type)
+    elif name.endswith(('.yaml', '.yml')):
+        from .cantera_yaml import read_mech_yaml
+        elems, specs, reacs = read_mech_yaml(name, sort_type)
     else:
         elems, specs, reacs = read_mech(mech_filename, therm_filename, sort_type)
     if not specs:

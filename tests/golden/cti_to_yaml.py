@@ -1,0 +1,124 @@
+"""Write Cantera-YAML copies of the CTI test mechanisms
+(tests/golden/mechanisms/{test,h2o2}.cti -> .yaml), with the same numbers in
+the same units, so that the YAML reader can be checked against the Chemkin
+and CTI inputs of the same mechanisms.  The CTI files are the reference's
+(pyjac/tests/test.cti, data/h2o2.cti); the YAML layout follows Cantera's
+published format (phases / species / reactions, NASA7 thermo, reaction
+types three-body, falloff, chemically-activated,
+pressure-dependent-Arrhenius, Chebyshev).
+
+    python tests/golden/cti_to_yaml.py
+"""
+
+import os
+import sys
+
+import yaml
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(HERE, '..', '..', 'pyjac-v2_amd'))
+
+from pyjac.core.cti import Call, parse_cti  # noqa: E402
+
+
+def _rate(k):
+    if isinstance(k, Call):
+        A, b, E = k.arg(0, 'A'), k.arg(1, 'b', 0.0), k.arg(2, 'E', 0.0)
+    else:
+        A, b, E = k[:3]
+    return {'A': float(A), 'b': float(b), 'Ea': float(E)}
+
+
+def _pressure(p):
+    return '{!r} {}'.format(float(p[0]), p[1]) if isinstance(p, list) else float(p)
+
+
+def _eff(c, d):
+    eff = c.kwargs.get('efficiencies')
+    if eff:
+        d['efficiencies'] = {t.split(':')[0]: float(t.split(':')[1]) for t in eff.split()}
+    if 'default_efficiency' in c.kwargs:
+        d['default-efficiency'] = float(c.kwargs['default_efficiency'])
+
+
+def _falloff(c, d):
+    fo = c.kwargs.get('falloff')
+    if isinstance(fo, Call) and fo.name in ('Troe', 'SRI'):
+        d[fo.name] = {k: float(v) for k, v in fo.kwargs.items()}
+
+
+def convert(text):
+    units, phase, species, reactions = None, None, [], []
+    for c in parse_cti(text):
+        if c.name == 'units':
+            units = {'length': c.kwargs.get('length', 'm'), 'time': 's',
+                     'quantity': c.kwargs.get('quantity', 'kmol'),
+                     'activation-energy': c.kwargs.get('act_energy', 'J/kmol')}
+        elif c.name == 'ideal_gas' and phase is None:
+            phase = {'name': c.kwargs.get('name', 'gas'), 'thermo': 'ideal-gas',
+                     'elements': c.kwargs['elements'].split(),
+                     'species': c.kwargs['species'].split(), 'kinetics': 'gas',
+                     'reactions': 'all'}
+        elif c.name == 'species':
+            lo, hi = sorted(c.kwargs['thermo'], key=lambda n: n.args[0][0])
+            comp = {}
+            for tok in c.kwargs['atoms'].split():
+                e, n = tok.split(':')
+                comp[e] = int(float(n))
+            species.append({'name': c.kwargs['name'], 'composition': comp,
+                            'thermo': {'model': 'NASA7',
+                                       'temperature-ranges': [float(lo.args[0][0]),
+                                                              float(lo.args[0][1]),
+                                                              float(hi.args[0][1])],
+                                       'data': [[float(x) for x in lo.args[1]],
+                                                [float(x) for x in hi.args[1]]]}})
+        elif c.name.endswith('reaction') or c.name == 'pdep_arrhenius':
+            d = {'equation': c.args[0]}
+            if c.name == 'reaction':
+                d['rate-constant'] = _rate(c.arg(1, 'kf'))
+            elif c.name == 'three_body_reaction':
+                d['type'] = 'three-body'
+                d['rate-constant'] = _rate(c.arg(1, 'kf'))
+                _eff(c, d)
+            elif c.name == 'falloff_reaction':
+                d['type'] = 'falloff'
+                d['low-P-rate-constant'] = _rate(c.arg(2, 'kf0'))
+                d['high-P-rate-constant'] = _rate(c.arg(1, 'kf'))
+                _falloff(c, d)
+                _eff(c, d)
+            elif c.name == 'chemically_activated_reaction':
+                d['type'] = 'chemically-activated'
+                d['low-P-rate-constant'] = _rate(c.arg(1, 'kLow'))
+                d['high-P-rate-constant'] = _rate(c.arg(2, 'kHigh'))
+                _falloff(c, d)
+                _eff(c, d)
+            elif c.name == 'pdep_arrhenius':
+                d['type'] = 'pressure-dependent-Arrhenius'
+                d['rate-constants'] = [dict(P=_pressure(r[0]), **_rate(r[1:])) for r in c.args[1:]]
+            elif c.name == 'chebyshev_reaction':
+                kw = c.kwargs
+                d['type'] = 'Chebyshev'
+                d['temperature-range'] = [float(kw['Tmin']), float(kw['Tmax'])]
+                d['pressure-range'] = [_pressure(kw['Pmin']), _pressure(kw['Pmax'])]
+                d['data'] = [[float(x) for x in row] for row in kw['coeffs']]
+            opts = c.kwargs.get('options', [])
+            if 'duplicate' in ([opts] if isinstance(opts, str) else opts):
+                d['duplicate'] = True
+            reactions.append(d)
+    doc = {'units': units, 'phases': [phase], 'species': species, 'reactions': reactions}
+    return yaml.safe_dump(doc, sort_keys=False, default_flow_style=None, width=100)
+
+
+def main():
+    mech = os.path.join(HERE, 'mechanisms')
+    for name in ('test', 'h2o2'):
+        with open(os.path.join(mech, name + '.cti')) as f:
+            text = convert(f.read())
+        out = os.path.join(mech, name + '.yaml')
+        with open(out, 'w') as f:
+            f.write('# written by tests/golden/cti_to_yaml.py from {}.cti\n'.format(name) + text)
+        print(out)
+
+
+if __name__ == '__main__':
+    main()

@@ -7,6 +7,13 @@
 // state, and ros2_stage2_kernel (k2, error estimate, accept or reject, next
 // step).  States that are done are skipped by the stage kernels; the host
 // stops when no state is active.
+//
+// The batch works on a compact copy of its states.  Whenever a sixteenth of
+// the states in it have finished, ros2_compact_kernel writes the finished
+// ones back to the caller's array (adding their counts to the statistics)
+// and packs the active ones into the other copy, so that the later attempts
+// -- often many more for the stiffest few states -- launch only over the
+// states still integrating.
 
 #include <hip/hip_runtime.h>
 
@@ -24,6 +31,20 @@ constexpr int NMAX = 120;      // largest NN (the iteration matrix in LDS)
 constexpr double GAMMA = 1.7071067811865475244;  // 1 + 1/sqrt(2)
 
 enum Status { ACTIVE = 0, DONE = 1, FAILED = 2 };
+
+// the per-state data that moves with a state when the batch is compacted
+struct StateSet {
+    double* y;       // [n][NN]
+    double* param;   // [n]
+    double* t;       // [n]
+    double* h;       // [n]
+    int* status;     // [n]
+    int* attempts;   // [n]
+    int* accepted;   // [n]
+    int* rejected;   // [n]
+    int* rejlast;    // [n]
+    int* orig;       // [n] the state's index in the caller's array
+};
 
 struct Chunk {
     long n;
@@ -250,20 +271,77 @@ __global__ void __launch_bounds__(NT) ros2_stage2_kernel(Chunk c, Params P) {
     }
 }
 
-__global__ void ros2_init_kernel(Chunk c, Params P) {
-    const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= c.n) return;
-    c.t[s] = 0.0;
-    c.h[s] = P.h0 > 0.0 ? fmin(P.h0, P.t_end) : 0.0;
-    c.status[s] = P.t_end > 0.0 ? ACTIVE : DONE;
-    c.attempts[s] = 0;
-    c.accepted[s] = 0;
-    c.rejected[s] = 0;
-    c.rejlast[s] = 0;
+// the working copy from the caller's states (one workgroup of 64 per state)
+__global__ void __launch_bounds__(64) ros2_init_kernel(StateSet a, long n, int N,
+                                                        const double* __restrict__ phi,
+                                                        const double* __restrict__ param,
+                                                        Params P) {
+    const long s = blockIdx.x;
+    if (s >= n) return;
+    for (int i = threadIdx.x; i < N; i += 64) a.y[s * N + i] = phi[s * N + i];
+    if (threadIdx.x == 0) {
+        a.param[s] = param[s];
+        a.t[s] = 0.0;
+        a.h[s] = P.h0 > 0.0 ? fmin(P.h0, P.t_end) : 0.0;
+        a.status[s] = P.t_end > 0.0 ? ACTIVE : DONE;
+        a.attempts[s] = 0;
+        a.accepted[s] = 0;
+        a.rejected[s] = 0;
+        a.rejlast[s] = 0;
+        a.orig[s] = (int)s;
+    }
+}
+
+// Finished states (every state with flush) go back to the caller's array and
+// into the statistics [accepted, rejected, failed, most attempts]; active
+// ones are packed into b at positions taken from *count.
+__global__ void __launch_bounds__(64) ros2_compact_kernel(StateSet a, StateSet b, long m, int N,
+                                                           double* __restrict__ phi, int* count,
+                                                           unsigned long long* stats, int flush) {
+    __shared__ long pos;
+    const long s = blockIdx.x;
+    if (s >= m) return;
+    const int st = a.status[s];
+    const bool keep = !flush && st == ACTIVE;
+    if (threadIdx.x == 0) {
+        if (keep) {
+            pos = atomicAdd(count, 1);
+        } else {
+            pos = -1;
+            atomicAdd(&stats[0], (unsigned long long)a.accepted[s]);
+            atomicAdd(&stats[1], (unsigned long long)a.rejected[s]);
+            if (st != DONE) atomicAdd(&stats[2], 1ull);
+            atomicMax(&stats[3], (unsigned long long)a.attempts[s]);
+        }
+    }
+    __syncthreads();
+    const double* src = a.y + s * N;
+    if (!keep) {
+        double* dst = phi + (long)a.orig[s] * N;
+        for (int i = threadIdx.x; i < N; i += 64) dst[i] = src[i];
+        return;
+    }
+    const long d = pos;
+    for (int i = threadIdx.x; i < N; i += 64) b.y[d * N + i] = src[i];
+    if (threadIdx.x == 0) {
+        b.param[d] = a.param[s];
+        b.t[d] = a.t[s];
+        b.h[d] = a.h[s];
+        b.status[d] = st;
+        b.attempts[d] = a.attempts[s];
+        b.accepted[d] = a.accepted[s];
+        b.rejected[d] = a.rejected[s];
+        b.rejlast[d] = a.rejlast[s];
+        b.orig[d] = a.orig[s];
+    }
 }
 
 __global__ void ros2_zero_kernel(int* p) {
     if (threadIdx.x == 0) p[0] = 0;
+}
+
+__global__ void ros2_zero_stats_kernel(unsigned long long* p) {
+    if (threadIdx.x < 4) p[threadIdx.x] = 0;
 }
 
 }  // namespace pjros
@@ -273,15 +351,40 @@ struct pyjac_integrator {
     int NN = 0;
     char* buf = nullptr;
     size_t buf_bytes = 0;
-    int* d_active = nullptr;
-    int* h_active = nullptr;  // pinned
+    int* d_counts = nullptr;                // [0] active after an attempt, [1] packing
+    unsigned long long* d_stats = nullptr;  // accepted, rejected, failed, most attempts
+    int* h_active = nullptr;                // pinned
 };
 
 namespace {
 
+// two copies of the moving per-state data and one of the per-attempt scratch
 size_t per_state_bytes(int NN) {
     const size_t N = NN;
-    return (N * N + 5 * N + 2) * sizeof(double) + (N + 5) * sizeof(int);
+    const size_t moving = (N + 3) * sizeof(double) + 6 * sizeof(int);
+    const size_t scratch = (N * N + 4 * N) * sizeof(double) + N * sizeof(int);
+    return 2 * moving + scratch;
+}
+
+char* carve(char*& p, size_t bytes) {
+    char* r = p;
+    p += (bytes + 255) & ~(size_t)255;
+    return r;
+}
+
+pjros::StateSet carve_set(char*& p, long cs, int N) {
+    pjros::StateSet a;
+    a.y = (double*)carve(p, (size_t)cs * N * sizeof(double));
+    a.param = (double*)carve(p, (size_t)cs * sizeof(double));
+    a.t = (double*)carve(p, (size_t)cs * sizeof(double));
+    a.h = (double*)carve(p, (size_t)cs * sizeof(double));
+    a.status = (int*)carve(p, (size_t)cs * sizeof(int));
+    a.attempts = (int*)carve(p, (size_t)cs * sizeof(int));
+    a.accepted = (int*)carve(p, (size_t)cs * sizeof(int));
+    a.rejected = (int*)carve(p, (size_t)cs * sizeof(int));
+    a.rejlast = (int*)carve(p, (size_t)cs * sizeof(int));
+    a.orig = (int*)carve(p, (size_t)cs * sizeof(int));
+    return a;
 }
 
 }  // namespace
@@ -300,7 +403,8 @@ int pyjac_hip_integrator_create(const pyjac_mech_tables* tables, int conp, pyjac
         delete it;
         return rc;
     }
-    if (hipMalloc(&it->d_active, sizeof(int)) != hipSuccess ||
+    if (hipMalloc(&it->d_counts, 2 * sizeof(int)) != hipSuccess ||
+        hipMalloc(&it->d_stats, 4 * sizeof(unsigned long long)) != hipSuccess ||
         hipHostMalloc(&it->h_active, sizeof(int), hipHostMallocDefault) != hipSuccess) {
         pyjac_hip_integrator_destroy(it);
         return PYJAC_ERR_ALLOC;
@@ -313,7 +417,8 @@ int pyjac_hip_integrator_destroy(pyjac_integrator* it) {
     if (!it) return PYJAC_OK;
     if (it->h) pyjac_hip_destroy(it->h);
     (void)hipFree(it->buf);
-    (void)hipFree(it->d_active);
+    (void)hipFree(it->d_counts);
+    (void)hipFree(it->d_stats);
     if (it->h_active) (void)hipHostFree(it->h_active);
     delete it;
     return PYJAC_OK;
@@ -329,12 +434,13 @@ int pyjac_hip_integrate_dev(pyjac_integrator* it, size_t n, const double* d_para
     const int N = it->NN;
     const size_t per = per_state_bytes(N);
     const long cs = (long)std::max<size_t>(1, std::min<size_t>(n, ((size_t)2 << 30) / per));
-    if (!it->buf || it->buf_bytes < (size_t)cs * per) {
+    const size_t need = (size_t)cs * per + 32 * 256;  // + the alignment of the carved arrays
+    if (!it->buf || it->buf_bytes < need) {
         (void)hipFree(it->buf);
         it->buf = nullptr;
         it->buf_bytes = 0;
-        if (hipMalloc(&it->buf, (size_t)cs * per) != hipSuccess) return PYJAC_ERR_ALLOC;
-        it->buf_bytes = (size_t)cs * per;
+        if (hipMalloc(&it->buf, need) != hipSuccess) return PYJAC_ERR_ALLOC;
+        it->buf_bytes = need;
     }
     const size_t lds = (size_t)N * N * sizeof(double);
     if (lds > 64 * 1024) {
@@ -344,44 +450,55 @@ int pyjac_hip_integrate_dev(pyjac_integrator* it, size_t n, const double* d_para
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
             return PYJAC_ERR_HIP;
     }
+    char* p = it->buf;
+    pjros::StateSet set[2] = {carve_set(p, cs, N), carve_set(p, cs, N)};
+    double* f0 = (double*)carve(p, (size_t)cs * N * sizeof(double));
+    double* J = (double*)carve(p, (size_t)cs * N * N * sizeof(double));
+    double* k1 = (double*)carve(p, (size_t)cs * N * sizeof(double));
+    double* y1 = (double*)carve(p, (size_t)cs * N * sizeof(double));
+    double* f1 = (double*)carve(p, (size_t)cs * N * sizeof(double));
+    int* piv = (int*)carve(p, (size_t)cs * N * sizeof(int));
+    if ((size_t)(p - it->buf) > it->buf_bytes) return PYJAC_ERR_ALLOC;
+
     pyjac_integrate_stats st{0, 0, 0, 0, 0};
     pjros::Params P{t_end, rtol, atol, h0, fixed, max_steps};
-    std::vector<int> h_att, h_acc, h_rej, h_status;
+    hipLaunchKernelGGL(pjros::ros2_zero_stats_kernel, dim3(1), dim3(64), 0, s, it->d_stats);
     for (size_t s0 = 0; s0 < n; s0 += cs) {
-        const long cnt = (long)std::min<size_t>(cs, n - s0);
-        pjros::Chunk c{};
-        c.n = cnt;
-        c.NN = N;
-        c.param = d_param + s0;
-        c.y = d_phi + s0 * N;
-        double* d = (double*)it->buf;
-        c.f0 = d;  d += (size_t)cnt * N;
-        c.J = d;   d += (size_t)cnt * N * N;
-        c.k1 = d;  d += (size_t)cnt * N;
-        c.y1 = d;  d += (size_t)cnt * N;
-        c.f1 = d;  d += (size_t)cnt * N;
-        c.t = d;   d += cnt;
-        c.h = d;   d += cnt;
-        int* ip = (int*)d;
-        c.piv = ip;      ip += (size_t)cnt * N;
-        c.status = ip;   ip += cnt;
-        c.attempts = ip; ip += cnt;
-        c.accepted = ip; ip += cnt;
-        c.rejected = ip; ip += cnt;
-        c.rejlast = ip;  ip += cnt;
-        c.active = it->d_active;
-        hipLaunchKernelGGL(pjros::ros2_init_kernel, dim3((unsigned)((cnt + 255) / 256)), dim3(256),
-                           0, s, c, P);
-        for (long att = 0; att < max_steps; ++att) {
-            int rc = pyjac_hip_species_rates_dev(it->h, cnt, c.param, c.y, c.f0, s);
-            if (!rc) rc = pyjac_hip_jacobian_dev(it->h, cnt, c.param, c.y, c.J, s);
+        long m = (long)std::min<size_t>(cs, n - s0);
+        double* phi = d_phi + s0 * N;
+        int cur = 0;
+        hipLaunchKernelGGL(pjros::ros2_init_kernel, dim3((unsigned)m), dim3(64), 0, s, set[0], m,
+                           N, phi, d_param + s0, P);
+        for (long att = 0; att < max_steps && m > 0; ++att) {
+            const pjros::StateSet& a = set[cur];
+            pjros::Chunk c{};
+            c.n = m;
+            c.NN = N;
+            c.param = a.param;
+            c.y = a.y;
+            c.f0 = f0;
+            c.J = J;
+            c.piv = piv;
+            c.k1 = k1;
+            c.y1 = y1;
+            c.f1 = f1;
+            c.t = a.t;
+            c.h = a.h;
+            c.status = a.status;
+            c.attempts = a.attempts;
+            c.accepted = a.accepted;
+            c.rejected = a.rejected;
+            c.rejlast = a.rejlast;
+            c.active = it->d_counts;
+            int rc = pyjac_hip_species_rates_dev(it->h, m, c.param, c.y, c.f0, s);
+            if (!rc) rc = pyjac_hip_jacobian_dev(it->h, m, c.param, c.y, c.J, s);
             if (rc) return rc;
-            hipLaunchKernelGGL(pjros::ros2_stage1_kernel, dim3((unsigned)cnt), dim3(pjros::NT),
+            hipLaunchKernelGGL(pjros::ros2_stage1_kernel, dim3((unsigned)m), dim3(pjros::NT),
                                lds, s, c, P);
-            rc = pyjac_hip_species_rates_dev(it->h, cnt, c.param, c.y1, c.f1, s);
+            rc = pyjac_hip_species_rates_dev(it->h, m, c.param, c.y1, c.f1, s);
             if (rc) return rc;
             hipLaunchKernelGGL(pjros::ros2_zero_kernel, dim3(1), dim3(64), 0, s, c.active);
-            hipLaunchKernelGGL(pjros::ros2_stage2_kernel, dim3((unsigned)cnt), dim3(pjros::NT),
+            hipLaunchKernelGGL(pjros::ros2_stage2_kernel, dim3((unsigned)m), dim3(pjros::NT),
                                lds, s, c, P);
             if (hipGetLastError() != hipSuccess) return PYJAC_ERR_HIP;
             ++st.launches;
@@ -389,24 +506,32 @@ int pyjac_hip_integrate_dev(pyjac_integrator* it, size_t n, const double* d_para
                     hipSuccess ||
                 hipStreamSynchronize(s) != hipSuccess)
                 return PYJAC_ERR_HIP;
-            if (*it->h_active == 0) break;
+            const long active = *it->h_active;
+            if (active == 0) break;
+            if (active <= m - std::max<long>(1, m / 16)) {
+                hipLaunchKernelGGL(pjros::ros2_zero_kernel, dim3(1), dim3(64), 0, s,
+                                   it->d_counts + 1);
+                hipLaunchKernelGGL(pjros::ros2_compact_kernel, dim3((unsigned)m), dim3(64), 0, s,
+                                   set[cur], set[1 - cur], m, N, phi, it->d_counts + 1,
+                                   it->d_stats, 0);
+                cur = 1 - cur;
+                m = active;
+            }
         }
-        h_att.resize(cnt);
-        h_acc.resize(cnt);
-        h_rej.resize(cnt);
-        h_status.resize(cnt);
-        if (hipMemcpy(h_att.data(), c.attempts, cnt * sizeof(int), hipMemcpyDeviceToHost) ||
-            hipMemcpy(h_acc.data(), c.accepted, cnt * sizeof(int), hipMemcpyDeviceToHost) ||
-            hipMemcpy(h_rej.data(), c.rejected, cnt * sizeof(int), hipMemcpyDeviceToHost) ||
-            hipMemcpy(h_status.data(), c.status, cnt * sizeof(int), hipMemcpyDeviceToHost))
-            return PYJAC_ERR_HIP;
-        for (long i = 0; i < cnt; ++i) {
-            st.accepted += h_acc[i];
-            st.rejected += h_rej[i];
-            st.max_steps = std::max<long>(st.max_steps, h_att[i]);
-            st.failed += h_status[i] != pjros::DONE;
-        }
+        // every state left goes back, finished or not
+        if (m > 0)
+            hipLaunchKernelGGL(pjros::ros2_compact_kernel, dim3((unsigned)m), dim3(64), 0, s,
+                               set[cur], set[1 - cur], m, N, phi, it->d_counts + 1, it->d_stats, 1);
+        if (hipGetLastError() != hipSuccess) return PYJAC_ERR_HIP;
     }
+    unsigned long long hs[4];
+    if (hipMemcpyAsync(hs, it->d_stats, sizeof(hs), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+        return PYJAC_ERR_HIP;
+    st.accepted = (long)hs[0];
+    st.rejected = (long)hs[1];
+    st.failed = (long)hs[2];
+    st.max_steps = (long)hs[3];
     if (stats) *stats = st;
     return PYJAC_OK;
 }

@@ -111,12 +111,21 @@ hipError_t launch_rates_block(const DevMech& m, const BlockArgs* args, const Blo
                               int conp, char order, long n, const double* param,
                               const double* phi, double* dphi, hipStream_t stream);
 
+// Second stream and events of the pipelined F-order path
+struct FPipeline {
+    hipStream_t aux;
+    hipEvent_t computed[2], transposed[2];
+};
+
 // F order with chunk_buf: C-order chunks of chunk_states states (a multiple
-// of 64) and a tiled transpose per chunk; otherwise written directly
+// of 64) and a tiled transpose per chunk; otherwise written directly.  With
+// pipe, chunk_buf holds two chunks and the transposes run on pipe->aux
+// beside the next chunk's Jacobians.
 hipError_t launch_jacobian_block(const DevMech& m, const BlockArgs* args, const BlockLds& L,
                                  int wps, int conp, char order, long n, const double* param,
                                  const double* phi, double* jac, double* chunk_buf,
-                                 long chunk_states, hipStream_t stream);
+                                 long chunk_states, hipStream_t stream,
+                                 const FPipeline* pipe = nullptr);
 
 // one workgroup of 64 * wps threads per state, Jacobian gathered in device
 // memory, chunk_states states (a multiple of 64) per launch: chunk_buf holds

@@ -1477,6 +1477,41 @@ entries_to_f_kernel(const double* __restrict__ src, long count, int nout,
     }
 }
 
+// The same transpose in 32 x 32 tiles (8.4 KB of LDS), small enough to run
+// beside the LDS-heavy Jacobian kernel on the same CUs: the pipelined F-order
+// path transposes one chunk while the next chunk's Jacobians are computed.
+// 16-byte pairs throughout (nout, n, s0 even).
+__global__ void __launch_bounds__(128)
+entries_to_f2_small_kernel(const double* __restrict__ src, long count, int nout,
+                           double* __restrict__ dst, long n, long s0) {
+    typedef double v2d __attribute__((ext_vector_type(2)));
+    __shared__ double tile[32][33];
+    const long t0 = (long)blockIdx.x * 32;
+    const int e0 = blockIdx.y * 32;
+    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;  // 8 rows per pass
+    for (int r = ty; r < 32; r += 8) {
+        const long st = t0 + r;
+        const int e = e0 + 2 * tx;
+        if (st < count && e < nout) {
+            const v2d v = *(const v2d*)(src + st * nout + e);  // nout even: e + 1 < nout
+            tile[r][2 * tx] = v.x;
+            tile[r][2 * tx + 1] = v.y;
+        }
+    }
+    __syncthreads();
+    for (int r = ty; r < 32; r += 8) {
+        const int e = e0 + r;
+        const long st = t0 + 2 * tx;
+        if (e < nout && st < count) {
+            if (st + 1 < count) {
+                *(v2d*)(dst + (size_t)e * n + s0 + st) = v2d{tile[2 * tx][r], tile[2 * tx + 1][r]};
+            } else {
+                dst[(size_t)e * n + s0 + st] = tile[2 * tx][r];
+            }
+        }
+    }
+}
+
 // Ragged batch: states of several mechanisms / option sets in one launch.
 // Member i owns workgroups [block0_i, block0_i + 8 ceil(n_i / 8)); inside
 // its range the XCD-aware state order of the single-mechanism launch applies.
@@ -1631,10 +1666,15 @@ hipError_t launch_rates_block(const DevMech& m, const BlockArgs* args, const Blo
 
 // chunk of cnt states' entries (C order) -> F-order output columns s0 ..
 static hipError_t launch_entries_to_f(const double* chunk, long cnt, int nout, double* jac, long n,
-                                      long s0, hipStream_t stream) {
+                                      long s0, hipStream_t stream, bool small = false) {
     dim3 grid((unsigned)((cnt + 63) / 64), (unsigned)((nout + 63) / 64));
-    if (nout % 2 == 0 && n % 2 == 0 && s0 % 2 == 0 && (uintptr_t)chunk % 16 == 0 &&
-        (uintptr_t)jac % 16 == 0)
+    const bool pairs = nout % 2 == 0 && n % 2 == 0 && s0 % 2 == 0 &&
+                       (uintptr_t)chunk % 16 == 0 && (uintptr_t)jac % 16 == 0;
+    if (pairs && small)
+        hipLaunchKernelGGL(pjb::entries_to_f2_small_kernel,
+                           dim3((unsigned)((cnt + 31) / 32), (unsigned)((nout + 31) / 32)),
+                           dim3(128), 0, stream, chunk, cnt, nout, jac, n, s0);
+    else if (pairs)
         hipLaunchKernelGGL(pjb::entries_to_f2_kernel, grid, dim3(256), 0, stream, chunk, cnt,
                            nout, jac, n, s0);
     else
@@ -1675,7 +1715,32 @@ static hipError_t launch_jacobian_block_chunk(const DevMech& m, const BlockArgs*
 hipError_t launch_jacobian_block(const DevMech& m, const BlockArgs* args, const BlockLds& BL,
                                  int wps, int conp, char order, long n, const double* param,
                                  const double* phi, double* jac, double* chunk_buf,
-                                 long chunk_states, hipStream_t stream) {
+                                 long chunk_states, hipStream_t stream, const FPipeline* pipe) {
+    if (order == 'F' && chunk_buf && pipe && n > chunk_states) {
+        // two chunk buffers: the transpose of chunk k (on the second stream)
+        // runs while chunk k + 1's Jacobians are computed; a buffer is
+        // reused once its transpose has finished, and the caller's stream
+        // waits for the last transpose
+        if (chunk_states < 64 || chunk_states % 64) return hipErrorInvalidValue;
+        const int nout = m.jac_nnz;
+        long k = 0;
+        for (long s0 = 0; s0 < n; s0 += chunk_states, ++k) {
+            const long cnt = std::min(chunk_states, n - s0);
+            const int b = (int)(k & 1);
+            double* buf = chunk_buf + (size_t)b * chunk_states * nout;
+            hipError_t e = hipSuccess;
+            if (k >= 2) e = hipStreamWaitEvent(stream, pipe->transposed[b], 0);
+            if (e == hipSuccess)
+                e = launch_jacobian_block_chunk(m, args, BL, wps, conp, n, s0, cnt, param, phi,
+                                                buf, stream);
+            if (e == hipSuccess) e = hipEventRecord(pipe->computed[b], stream);
+            if (e == hipSuccess) e = hipStreamWaitEvent(pipe->aux, pipe->computed[b], 0);
+            if (e == hipSuccess) e = launch_entries_to_f(buf, cnt, nout, jac, n, s0, pipe->aux, true);
+            if (e == hipSuccess) e = hipEventRecord(pipe->transposed[b], pipe->aux);
+            if (e != hipSuccess) return e;
+        }
+        return hipStreamWaitEvent(stream, pipe->transposed[(k - 1) & 1], 0);
+    }
     if (order == 'F' && chunk_buf) {
         // F order through C-order chunks: each state's entries are written as
         // one contiguous run, then a tiled transpose writes whole lines

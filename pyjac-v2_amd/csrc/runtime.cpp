@@ -77,9 +77,12 @@ struct pyjac_handle {
     int wps_g = 0;
     double* d_chunk = nullptr;  // F order: one chunk of states' entries
     long chunk_states = 0;
-    // LDS-kernel Jacobian in F order: C-order chunk before the transpose
+    // LDS-kernel Jacobian in F order: C-order chunks before the transpose,
+    // two of them when the transposes are pipelined on a second stream
     double* d_fchunk = nullptr;
     long fchunk_states = 0;
+    bool fpipe_ok = false;
+    pjk_host::FPipeline fpipe{};
     // finite-difference Jacobian: options, coefficients (device), buffers
     int fd_mode = PYJAC_FD_FORWARD, fd_order = 1, fd_K = 0;
     double fd_rtol = 1e-8, fd_atol = 1e-15;
@@ -206,14 +209,25 @@ long fchunk_want(const pyjac_handle* h, long n) {
     return std::max(64L, std::min(want, (n + 63) / 64 * 64));
 }
 
+// two chunks and the second stream (default priority, non-blocking) with
+// its events, made once per handle
 int ensure_fchunk(pyjac_handle* h, long n) {
     const long want = fchunk_want(h, n);
     if (!want || (h->d_fchunk && h->fchunk_states >= want)) return PYJAC_OK;
     (void)hipFree(h->d_fchunk);
     h->d_fchunk = nullptr;
     h->fchunk_states = 0;
-    HIP_TRY(hipMalloc(&h->d_fchunk, (size_t)want * h->m.jac_nnz * sizeof(double)));
+    HIP_TRY(hipMalloc(&h->d_fchunk, 2 * (size_t)want * h->m.jac_nnz * sizeof(double)));
     h->fchunk_states = want;
+    if (!h->fpipe_ok) {
+        pjk_host::FPipeline& p = h->fpipe;
+        HIP_TRY(hipStreamCreateWithFlags(&p.aux, hipStreamNonBlocking));
+        for (int i = 0; i < 2; ++i) {
+            HIP_TRY(hipEventCreateWithFlags(&p.computed[i], hipEventDisableTiming));
+            HIP_TRY(hipEventCreateWithFlags(&p.transposed[i], hipEventDisableTiming));
+        }
+        h->fpipe_ok = true;
+    }
     return PYJAC_OK;
 }
 
@@ -427,7 +441,8 @@ int run_jac(pyjac_handle* h, long n, const double* param, const double* phi, dou
         HIP_TRY(pjk_host::launch_jacobian_block(h->m, h->d_args[1], h->lds[1], block_wps(h, 1),
                                                 h->conp, h->order, n, param, phi, jac,
                                                 chunked ? h->d_fchunk : nullptr,
-                                                chunked ? h->fchunk_states : 0, s));
+                                                chunked ? h->fchunk_states : 0, s,
+                                                chunked && h->fpipe_ok ? &h->fpipe : nullptr));
         return PYJAC_OK;
     }
     if (use_glob(h)) {
@@ -1076,6 +1091,13 @@ int pyjac_hip_destroy(pyjac_handle* h) {
     (void)hipFree(h->d_args_g);
     (void)hipFree(h->d_chunk);
     (void)hipFree(h->d_fchunk);
+    if (h->fpipe_ok) {
+        (void)hipStreamDestroy(h->fpipe.aux);
+        for (int i = 0; i < 2; ++i) {
+            (void)hipEventDestroy(h->fpipe.computed[i]);
+            (void)hipEventDestroy(h->fpipe.transposed[i]);
+        }
+    }
     (void)hipFree(h->d_fd);
     (void)hipFree(h->d_fd_x);
     (void)hipFree(h->d_fd_y);

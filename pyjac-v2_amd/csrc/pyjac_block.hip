@@ -185,29 +185,55 @@ __device__ __forceinline__ Ctx make_ctx(const BlockArgs* A, double* smem, int co
 // ------------------------------------------------------------- phase 1
 // get_concentrations (rate_subs.py:597-706) and the NASA-7 properties
 // (rate_subs.py:3281-3388); `phi(i)` reads entry i of this state
-template <bool JAC, int NT, class PhiF>
+// species k's energy (h or u), heat capacity (cp or cv) and its T
+// derivative from the NASA-7 polynomials (rate_subs.py:3281-3388)
+struct Thermo {
+    double energy, heat, dheat;
+};
+
+__device__ __forceinline__ const double* nasa_coeffs(const DevMech& m, double T, int k) {
+    return (T < m.T_mid[k]) ? m.a_lo + 7 * k : m.a_hi + 7 * k;
+}
+
+__device__ __forceinline__ Thermo nasa_thermo(const double* a, double T, int conp) {
+    const double a0 = a[0], a1 = a[1], a2 = a[2], a3 = a[3], a4 = a[4], a5 = a[5];
+    const double x = T * (T * (T * (T * (T * a4 / 5.0 + a3 / 4.0) + a2 / 3.0) + a1 / 2.0) + a0);
+    const double y = T * (T * (T * (T * a4 + a3) + a2) + a1) + a0;
+    Thermo h;
+    h.energy = RU * (conp ? x + a5 : x - T + a5);
+    h.heat = RU * (conp ? y : y - 1.0);
+    h.dheat = RU * (T * (T * (4.0 * T * a4 + 3.0 * a3) + 2.0 * a2) + a1);
+    return h;
+}
+
+// With LATE, the dense Jacobian kernel's layout: no moles array (the bath
+// gas sums phi itself) and the energies, heat capacities and their
+// derivatives are recomputed after the reaction pass (they share LDS with
+// arrays that only the reaction pass reads).
+template <bool JAC, int NT, bool LATE = false, class PhiF>
 __device__ __forceinline__ void phase_species(Ctx& c, const State& st, int tid, PhiF phi) {
     const DevMech& m = c.m;
     const int Ns = m.Ns;
     for (int k = tid; k < Ns; k += NT) {
         if (k < Ns - 1) {
             double n = fmax(SMALL, phi(2 + k));
-            c.nmol(k) = n;
+            if (!LATE) c.nmol(k) = n;
             c.conc(k) = n * st.V_inv;
         }
         const double T = st.Tc;
-        const double* a = (T < m.T_mid[k]) ? m.a_lo + 7 * k : m.a_hi + 7 * k;
+        const double* a = nasa_coeffs(m, T, k);
         const double a0 = a[0], a1 = a[1], a2 = a[2], a3 = a[3], a4 = a[4], a5 = a[5], a6 = a[6];
         c.b(k) = T * (T * (T * (T * a4 / 20.0 + a3 / 12.0) + a2 / 6.0) + a1 / 2.0) +
                  (a0 + -1.0) * st.logT + -1.0 * a0 + a6 + -1.0 * a5 * st.Tinv;
-        const double x = T * (T * (T * (T * (T * a4 / 5.0 + a3 / 4.0) + a2 / 3.0) + a1 / 2.0) + a0);
-        const double y = T * (T * (T * (T * a4 + a3) + a2) + a1) + a0;
-        c.energy(k) = RU * (c.conp ? x + a5 : x - T + a5);
-        c.heat(k) = RU * (c.conp ? y : y - 1.0);
+        if (!LATE) {
+            const Thermo h = nasa_thermo(a, T, c.conp);
+            c.energy(k) = h.energy;
+            c.heat(k) = h.heat;
+            if (JAC) c.dheat(k) = h.dheat;
+        }
         c.wdot(k) = 0.0;
         if (JAC) {
             c.U(k) = 0.0;
-            c.dheat(k) = RU * (T * (T * (4.0 * T * a4 + 3.0 * a3) + 2.0 * a2) + a1);
             c.db(k) = T * (T * (T * a4 / 5.0 + a3 / 4.0) + a2 / 3.0) + a1 / 2.0 +
                       st.Tinv * (a0 + -1.0 + a5 * st.Tinv);
         }
@@ -221,6 +247,16 @@ __device__ __forceinline__ void bath_gas(Ctx& c, const State& st, int lane) {
     const int Ns = c.m.Ns;
     double nsum = 0.0;
     for (int k = 0; k < Ns - 1; ++k) nsum = nsum + c.nmol(k);
+    if (lane == 0)
+        c.conc(Ns - 1) = __dadd_rn(st.P / (RU * st.Tc), __dmul_rn(-1.0 * fmax(SMALL, nsum), st.V_inv));
+}
+
+// the same sum over the floored moles read from the state itself
+template <class PhiF>
+__device__ __forceinline__ void bath_gas_phi(Ctx& c, const State& st, int lane, PhiF phi) {
+    const int Ns = c.m.Ns;
+    double nsum = 0.0;
+    for (int k = 0; k < Ns - 1; ++k) nsum = nsum + fmax(SMALL, phi(2 + k));
     if (lane == 0)
         c.conc(Ns - 1) = __dadd_rn(st.P / (RU * st.Tc), __dmul_rn(-1.0 * fmax(SMALL, nsum), st.V_inv));
 }
@@ -955,8 +991,10 @@ __device__ __forceinline__ void jacobian_state(const BlockArgs* __restrict__ A, 
     const int Ns = c.Ns, NN = c.NN, ns = c.ns, Nr = c.m.Nr, nout = c.m.jac_nnz;
     lds_double* J = c.J();
     const JacAcc acc{J, nullptr, nullptr, nullptr};
+    // rows 0 and 1 of the accumulator hold the Gibbs terms b and their T
+    // derivatives db until the reaction pass is over (they are first
+    // written in phase 5), so only rows 2 .. NN - 1 are cleared here
     if (tid == 0) { c.conc(Ns) = 1.0; c.b(Ns) = 0.0; c.db(Ns) = 0.0; }
-    for (int k = tid; k < Ns - 1; k += NT) c.mw(k) = c.m.mw_post[k];
     const double* ps = phi + s * L.sN;
     auto ph = [&](int i) { return ps[i * L.sI]; };
     const State st = load_state(conp, param[s], ph(0), ph(1));
@@ -965,13 +1003,13 @@ __device__ __forceinline__ void jacobian_state(const BlockArgs* __restrict__ A, 
         typedef __attribute__((address_space(3))) v2d lds_v2d;
         lds_v2d* J2 = (lds_v2d*)J;
         const int half = (NN * NN) / 2;
-        for (int e = tid; e < half; e += NT) J2[e] = v2d{0.0, 0.0};
+        for (int e = NN + tid; e < half; e += NT) J2[e] = v2d{0.0, 0.0};
         if (tid == 0 && (NN * NN) % 2) J[NN * NN - 1] = 0.0;
         for (int p = tid; p < c.bt.n_pairs; p += NT) c.Cp(p) = 0.0;
     }
-    phase_species<true, NT>(c, st, tid, ph);
+    phase_species<true, NT, true>(c, st, tid, ph);
     __syncthreads();
-    bath_gas(c, st, lane);
+    bath_gas_phi(c, st, lane, ph);
     phase_pdep<true, WPS>(c, st, wave, lane);
     __syncthreads();
     // phase 3: reactions
@@ -997,13 +1035,23 @@ __device__ __forceinline__ void jacobian_state(const BlockArgs* __restrict__ A, 
             lds_add(J + e.y, c.Cp(e.x) * __hiloint2double(e.w, e.z));
         }
     }
-    // temperature rate and total heat capacity (every wave)
+    // b and db are spent: clear rows 0 and 1 for phase 5
+    for (int e = tid; e < 2 * NN; e += NT) J[e] = 0.0;
+    // temperature rate and total heat capacity (every wave), with the
+    // species' thermodynamic properties recomputed here; wave 0 keeps them
+    // in LDS (over the third-body arrays of the reaction pass) for phase 5
     double heat_tot, Tdot;
     {
         double lo = 0.0, up = 0.0;
         for (int k = lane; k < Ns; k += 64) {
-            lo += c.conc(k) * c.heat(k);
-            up += c.energy(k) * c.wdot(k);
+            const Thermo h = nasa_thermo(nasa_coeffs(c.m, st.Tc, k), st.Tc, conp);
+            lo += c.conc(k) * h.heat;
+            up += h.energy * c.wdot(k);
+            if (wave == 0) {
+                c.energy(k) = h.energy;
+                c.heat(k) = h.heat;
+                c.dheat(k) = h.dheat;
+            }
         }
         heat_tot = wave_sum(lo);
         Tdot = 0.0 + -1.0 * wave_sum(up) / heat_tot;
@@ -1024,7 +1072,7 @@ __device__ __forceinline__ void jacobian_state(const BlockArgs* __restrict__ A, 
             double sum = 0.0, sum2 = 0.0;
             for (int k = k0; k < k1; ++k) {
                 const double jv = J[(k + 2) * NN + j + 2] + c.U(k);
-                const double mw = c.mw(k);
+                const double mw = sld(c.m.mw_post, k);
                 sum = sum + (c.energy(k) + -1.0 * e_ns * mw) * jv;
                 sum2 = sum2 + (1.0 + -1.0 * mw) * jv;
             }
@@ -1048,7 +1096,7 @@ __device__ __forceinline__ void jacobian_state(const BlockArgs* __restrict__ A, 
         for (int i = lane; i < Ns - 1; i += 64) {
             const double j0 = J[(i + 2) * NN];
             const double j1 = J[(i + 2) * NN + 1];
-            const double mw = c.mw(i);
+            const double mw = c.m.mw_post[i];
             const double w = c.wdot(i);
             rate_sum += Vinv * j0 * (-1.0 * c.energy(i) + e_ns * mw) +
                         w * (-1.0 * c.heat(i) + mw * h_ns);
@@ -1601,6 +1649,38 @@ BlockLds block_lds(const DevMech& m, bool jac, int bt_pairs, bool dense, int bt_
         L.total = o;
         return L;
     }
+    if (jac) {
+        // dense accumulator: the arrays only the pressure-dependent setup
+        // and the reaction pass read (third-body factors, PLOG/Chebyshev
+        // rate constants) are reused afterwards for the energies, heat
+        // capacities and their derivatives; the Gibbs terms b and db lie in
+        // rows 0 and 1 of the accumulator until phase 5 writes those rows;
+        // no moles or molecular weights in LDS
+        L.conc = take(Ns + 1);
+        L.wdot = take(Ns + 1);
+        L.U = take(Ns);
+        L.Cp = take(bt_pairs);
+        const int r0 = o;
+        L.ci = take(m.n_thd);
+        L.ksp = take(m.n_plog + m.n_cheb);
+        L.nfac = take(m.n_thd);
+        L.dTf = take(m.n_thd);
+        L.dEf = take(m.n_thd);
+        L.dkT = take(m.n_plog + m.n_cheb);
+        L.dkP = take(m.n_plog + m.n_cheb);
+        o = r0 + std::max(o - r0, 3 * Ns);
+        L.energy = r0;
+        L.heat = r0 + Ns;
+        L.dheat = r0 + 2 * Ns;
+        L.nmol = -1;
+        L.mw = -1;
+        o = (o + 1) & ~1;  // 16-byte alignment of the accumulator
+        L.jac = take(NN * NN);
+        L.b = L.jac;
+        L.db = L.jac + NN;
+        L.total = o;
+        return L;
+    }
     L.conc = take(Ns + 1);
     L.nmol = take(Ns);
     L.b = take(Ns + 1);
@@ -1609,20 +1689,6 @@ BlockLds block_lds(const DevMech& m, bool jac, int bt_pairs, bool dense, int bt_
     L.heat = take(Ns);
     L.ci = take(m.n_thd);
     L.ksp = take(m.n_plog + m.n_cheb);
-    if (jac) {
-        L.db = take(Ns + 1);
-        L.dheat = take(Ns);
-        L.U = take(Ns);
-        L.nfac = take(m.n_thd);
-        L.dTf = take(m.n_thd);
-        L.dEf = take(m.n_thd);
-        L.Cp = take(bt_pairs);
-        L.mw = take(Ns - 1);
-        L.dkT = take(m.n_plog + m.n_cheb);
-        L.dkP = take(m.n_plog + m.n_cheb);
-        o = (o + 1) & ~1;  // 16-byte alignment of the accumulator
-        L.jac = take(m.NN * m.NN);
-    }
     L.total = o;
     return L;
 }

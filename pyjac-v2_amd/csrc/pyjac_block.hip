@@ -270,8 +270,53 @@ __device__ __forceinline__ void bath_gas_phi(Ctx& c, const State& st, int lane, 
 //        (create_jacobian.py:3780-3890; 1 for plain third bodies)
 //   dTf, dEf  d(ci)/dT, d(ci)/dE per unit rate of progress
 //        (create_jacobian.py:362-882 and :2117-2588)
+// The parts of a falloff entry that depend on the temperature only: the
+// rate constants of both limits, the Troe centre Fcent with its logarithms
+// and (Jacobian) its T derivative.  The dense Jacobian kernel evaluates them
+// in phase 1, on threads the species leave idle, and the rest of the entry
+// after the barrier.
+struct ThdPre {
+    double kf, kff, Fc, logFc, lnFc, dFcent;
+};
+
 template <bool JAC>
-__device__ void thd_task(Ctx& c, const State& st, int t) {
+__device__ __forceinline__ ThdPre thd_pre(const Ctx& c, const State& st, int t) {
+    const DevMech& m = c.m;
+    ThdPre p{0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    const int r = m.thd_map[t];
+    const int f = m.fall_mask[r];
+    if (f < 0) return p;
+    const int si = m.simple_mask[r];
+    p.kf = arrhenius(m.simple_form[si], m.simple_A[si], m.simple_b[si], m.simple_Ta[si], st.Tc,
+                     st.logT, st.Tinv);
+    p.kff = arrhenius(m.fall_form[f], m.fall_A[f], m.fall_b[f], m.fall_Ta[f], st.Tc, st.logT,
+                      st.Tinv);
+    if (m.fall_blend[f] == 2) {
+        const int i = m.fall_param[f];
+        const double a = m.troe_a[i];
+        double Fc = a * exp(fmin(EXP_MAX, -1.0 * st.Tc * m.troe_T1inv[i])) +
+                    (1.0 + -1.0 * a) * exp(fmin(EXP_MAX, -1.0 * st.Tc * m.troe_T3inv[i]));
+        const double T2 = m.troe_T2[i];
+        if (T2 != 0.0) Fc = Fc + exp(fmin(EXP_MAX, -1.0 * T2 / st.Tc));
+        p.Fc = Fc;
+        p.logFc = log10(fmax(SMALL, Fc));
+        if (JAC) {
+            p.lnFc = log(fmax(SMALL, Fc));
+            const double T = st.T, Tinv = st.Traw_inv;
+            const double T1 = m.troe_T1inv[i], T3 = m.troe_T3inv[i];
+            p.dFcent = -1.0 * a * T1 * exp(fmin(EXP_MAX, -1.0 * T * T1)) +
+                       (a + -1.0) * T3 * exp(fmin(EXP_MAX, -1.0 * T * T3)) +
+                       T2 * Tinv * Tinv * exp(fmin(EXP_MAX, -1.0 * T2 * Tinv));
+        }
+    }
+    return p;
+}
+
+template <bool JAC>
+__device__ void thd_task(Ctx& c, const State& st, int t);
+
+template <bool JAC>
+__device__ void thd_post(Ctx& c, const State& st, int t, const ThdPre& pre) {
     const DevMech& m = c.m;
     const int conp = c.conp;
     const int tt = m.thd_type[t];
@@ -302,10 +347,7 @@ __device__ void thd_task(Ctx& c, const State& st, int t) {
     }
     // falloff / chemically activated
     const int si = m.simple_mask[r];
-    const double kf = arrhenius(m.simple_form[si], m.simple_A[si], m.simple_b[si], m.simple_Ta[si],
-                                st.Tc, st.logT, st.Tinv);
-    const double kff = arrhenius(m.fall_form[f], m.fall_A[f], m.fall_b[f], m.fall_Ta[f], st.Tc,
-                                 st.logT, st.Tinv);
+    const double kf = pre.kf, kff = pre.kff;
     const int ftype = m.fall_type[f];
     const double kinf = ftype ? kff : kf;
     const double k0 = ftype ? kf : kff;
@@ -313,14 +355,9 @@ __device__ void thd_task(Ctx& c, const State& st, int t) {
     const int blend = m.fall_blend[f];
     double Fi = 1.0, Fc = 0.0, A = 0.0, B = 0.0, X = 0.0;
     if (blend == 2) {
-        const int i = m.fall_param[f];
-        const double a = m.troe_a[i];
         const double logPr = log10(fmax(SMALL, Pr));
-        Fc = a * exp(fmin(EXP_MAX, -1.0 * st.Tc * m.troe_T1inv[i])) +
-             (1.0 + -1.0 * a) * exp(fmin(EXP_MAX, -1.0 * st.Tc * m.troe_T3inv[i]));
-        const double T2 = m.troe_T2[i];
-        if (T2 != 0.0) Fc = Fc + exp(fmin(EXP_MAX, -1.0 * T2 / st.Tc));
-        const double logFc = log10(fmax(SMALL, Fc));
+        Fc = pre.Fc;
+        const double logFc = pre.logFc;
         B = -1.1762 * logFc + -1.0 * 0.14 * logPr + 0.806;
         A = -0.67 * logFc + logPr + -0.4;
         Fi = pow(fmax(SMALL, Fc), 1.0 / (A * A / (B * B) + 1.0));
@@ -345,7 +382,7 @@ __device__ void thd_task(Ctx& c, const State& st, int t) {
         double dFi = 0.0;
         if (blend == 2) {
             dFi = A * A + B * B;
-            dFi = -2.0 * A * B * (0.14 * A + B) * log(fmax(SMALL, Fc)) /
+            dFi = -2.0 * A * B * (0.14 * A + B) * pre.lnFc /
                   (fmax(SMALL, Pr) * dFi * dFi * LN10);
         } else if (blend == 3) {
             const int i = m.fall_param[f];
@@ -372,13 +409,8 @@ __device__ void thd_task(Ctx& c, const State& st, int t) {
         const double theta_no_Pr = thd_fac * k0 / kinf;
         double dFi = 0.0;
         if (blend == 2) {
-            const int i = m.fall_param[f];
-            const double a = m.troe_a[i], T1 = m.troe_T1inv[i], T2 = m.troe_T2[i],
-                         T3 = m.troe_T3inv[i];
-            const double dFcent = -1.0 * a * T1 * exp(fmin(EXP_MAX, -1.0 * T * T1)) +
-                                  (a + -1.0) * T3 * exp(fmin(EXP_MAX, -1.0 * T * T3)) +
-                                  T2 * Tinv * Tinv * exp(fmin(EXP_MAX, -1.0 * T2 * Tinv));
-            const double logFcent = log(fmax(SMALL, Fc));
+            const double dFcent = pre.dFcent;
+            const double logFcent = pre.lnFc;
             const double absq = A * A + B * B;
             const double absqsq = absq * absq;
             dFi = -1.0 * B *
@@ -410,7 +442,7 @@ __device__ void thd_task(Ctx& c, const State& st, int t) {
         if (blend == 2) {
             double absqsq = A * A + B * B;
             absqsq = absqsq * absqsq;
-            dFi = -2.0 * A * B * log(fmax(SMALL, Fc)) * (0.14 * A + B) * theta /
+            dFi = -2.0 * A * B * pre.lnFc * (0.14 * A + B) * theta /
                   (fmax(SMALL, Pr) * absqsq * LN10);
         } else if (blend == 3) {
             const int i = m.fall_param[f];
@@ -436,6 +468,11 @@ __device__ void thd_task(Ctx& c, const State& st, int t) {
         }
         c.dEf(t) = conp ? d : d * st.V;
     }
+}
+
+template <bool JAC>
+__device__ void thd_task(Ctx& c, const State& st, int t) {
+    thd_post<JAC>(c, st, t, thd_pre<JAC>(c, st, t));
 }
 
 __device__ __forceinline__ double plog_par(const DevMech& m, int q, int p, int k) {
@@ -1007,10 +1044,28 @@ __device__ __forceinline__ void jacobian_state(const BlockArgs* __restrict__ A, 
         if (tid == 0 && (NN * NN) % 2) J[NN * NN - 1] = 0.0;
         for (int p = tid; p < c.bt.n_pairs; p += NT) c.Cp(p) = 0.0;
     }
+    // pressure-dependent entries: when the species leave enough threads
+    // idle in phase 1 (GRI: 53 species, 41 entries, 128 threads), the last
+    // threads evaluate the temperature-only parts of the falloff entries and
+    // the whole PLOG / Chebyshev entries there, and finish the falloff
+    // entries after the barrier
+    const int n1 = c.m.n_thd, n2 = n1 + c.m.n_plog, n3 = n2 + c.m.n_cheb;
+    const bool early = Ns + n3 <= NT;
+    const int task = tid - (NT - n3);
+    ThdPre pre{0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     phase_species<true, NT, true>(c, st, tid, ph);
+    if (early && task >= 0) {
+        if (task < n1) pre = thd_pre<true>(c, st, c.bt.thd_order[task]);
+        else if (task < n2) plog_task<true>(c, st, task - n1);
+        else cheb_task<true>(c, st, task - n2);
+    }
     __syncthreads();
     bath_gas_phi(c, st, lane, ph);
-    phase_pdep<true, WPS>(c, st, wave, lane);
+    if (early) {
+        if (task >= 0 && task < n1) thd_post<true>(c, st, c.bt.thd_order[task], pre);
+    } else {
+        phase_pdep<true, WPS>(c, st, wave, lane);
+    }
     __syncthreads();
     // phase 3: reactions
     for (int pass = 0; pass * NT < Nr; ++pass) {

@@ -315,8 +315,11 @@ __device__ __forceinline__ ThdPre thd_pre(const Ctx& c, const State& st, int t) 
 template <bool JAC>
 __device__ void thd_task(Ctx& c, const State& st, int t);
 
-template <bool JAC>
-__device__ void thd_post(Ctx& c, const State& st, int t, const ThdPre& pre) {
+// get_pre() yields the entry's ThdPre, evaluated on the spot or kept from
+// phase 1; it is called after the third-body sum, where the rate constants
+// are first needed
+template <bool JAC, class PreF>
+__device__ void thd_post(Ctx& c, const State& st, int t, PreF get_pre) {
     const DevMech& m = c.m;
     const int conp = c.conp;
     const int tt = m.thd_type[t];
@@ -324,9 +327,10 @@ __device__ void thd_post(Ctx& c, const State& st, int t, const ThdPre& pre) {
     double thd = st.P * (double)not_spec / (RU * st.T);
     const int e1 = m.thd_off[t + 1];
     int s = m.thd_off[t];
-    // four efficiencies per step with their loads issued together; the sum
-    // keeps its sequential order
-    for (; s + 4 <= e1; s += 4) {
+    // four efficiencies per step with their loads issued together (the
+    // Jacobian kernels; the species-rate kernel has no registers to spare);
+    // the sum keeps its sequential order
+    for (; JAC && s + 4 <= e1; s += 4) {
         const double f0 = m.thd_eff[s], f1 = m.thd_eff[s + 1], f2 = m.thd_eff[s + 2],
                      f3 = m.thd_eff[s + 3];
         const int k0 = m.thd_spec[s], k1 = m.thd_spec[s + 1], k2 = m.thd_spec[s + 2],
@@ -360,6 +364,7 @@ __device__ void thd_post(Ctx& c, const State& st, int t, const ThdPre& pre) {
         return;
     }
     // falloff / chemically activated
+    const ThdPre pre = get_pre();
     const int si = m.simple_mask[r];
     const double kf = pre.kf, kff = pre.kff;
     const int ftype = m.fall_type[f];
@@ -486,7 +491,7 @@ __device__ void thd_post(Ctx& c, const State& st, int t, const ThdPre& pre) {
 
 template <bool JAC>
 __device__ void thd_task(Ctx& c, const State& st, int t) {
-    thd_post<JAC>(c, st, t, thd_pre<JAC>(c, st, t));
+    thd_post<JAC>(c, st, t, [&]() { return thd_pre<JAC>(c, st, t); });
 }
 
 __device__ __forceinline__ double plog_par(const DevMech& m, int q, int p, int k) {
@@ -1076,7 +1081,8 @@ __device__ __forceinline__ void jacobian_state(const BlockArgs* __restrict__ A, 
     __syncthreads();
     bath_gas_phi(c, st, lane, ph);
     if (early) {
-        if (task >= 0 && task < n1) thd_post<true>(c, st, c.bt.thd_order[task], pre);
+        if (task >= 0 && task < n1)
+            thd_post<true>(c, st, c.bt.thd_order[task], [&]() { return pre; });
     } else {
         phase_pdep<true, WPS>(c, st, wave, lane);
     }

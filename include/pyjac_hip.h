@@ -198,7 +198,10 @@ int pyjac_hip_chem_utils(pyjac_handle *h, const double *phi,
 
 /* Device-pointer entry points: n_states states already resident in device
    memory in the handle's data order, launched on `stream` (a hipStream_t,
-   NULL for the default stream), asynchronous. */
+   NULL for the default stream), asynchronous.  A handle's calls must not
+   overlap in time (they share its work buffers and, for F-order Jacobians,
+   its second stream): serialise them on one stream, or use one handle per
+   concurrent stream. */
 int pyjac_hip_species_rates_dev(pyjac_handle *h, size_t n_states,
                                 const double *d_param, const double *d_phi,
                                 double *d_dphi, void *stream);

@@ -188,7 +188,10 @@ def gpu_reactor(mech, rtol=1e-9, atol=1e-15):
     def react(T, P, Y, dt):
         phi = torch.from_numpy(_to_phi(thermo, T, P, Y)).cuda()
         param = torch.full((phi.shape[0],), float(P), dtype=torch.float64, device='cuda')
-        integ.integrate(param, phi, dt, rtol=rtol, atol=atol)
+        stats = integ.integrate(param, phi, dt, rtol=rtol, atol=atol)
+        if stats['failed']:
+            raise RuntimeError('{} particle(s) did not reach the end of the reaction step'
+                               .format(stats['failed']))
         return _from_phi(thermo, P, phi.cpu().numpy())
     return react
 

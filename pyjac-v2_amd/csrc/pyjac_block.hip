@@ -291,7 +291,8 @@ __device__ __forceinline__ ThdPre thd_pre(const Ctx& c, const State& st, int t) 
                      st.logT, st.Tinv);
     p.kff = arrhenius(m.fall_form[f], m.fall_A[f], m.fall_b[f], m.fall_Ta[f], st.Tc, st.logT,
                       st.Tinv);
-    if (m.fall_blend[f] == 2) {
+    // the species-rate kernel evaluates the Troe centre where it is used
+    if (JAC && m.fall_blend[f] == 2) {
         const int i = m.fall_param[f];
         const double a = m.troe_a[i];
         double Fc = a * exp(fmin(EXP_MAX, -1.0 * st.Tc * m.troe_T1inv[i])) +
@@ -375,8 +376,19 @@ __device__ void thd_post(Ctx& c, const State& st, int t, PreF get_pre) {
     double Fi = 1.0, Fc = 0.0, A = 0.0, B = 0.0, X = 0.0;
     if (blend == 2) {
         const double logPr = log10(fmax(SMALL, Pr));
-        Fc = pre.Fc;
-        const double logFc = pre.logFc;
+        double logFc;
+        if constexpr (JAC) {
+            Fc = pre.Fc;
+            logFc = pre.logFc;
+        } else {
+            const int i = m.fall_param[f];
+            const double a = m.troe_a[i];
+            Fc = a * exp(fmin(EXP_MAX, -1.0 * st.Tc * m.troe_T1inv[i])) +
+                 (1.0 + -1.0 * a) * exp(fmin(EXP_MAX, -1.0 * st.Tc * m.troe_T3inv[i]));
+            const double T2 = m.troe_T2[i];
+            if (T2 != 0.0) Fc = Fc + exp(fmin(EXP_MAX, -1.0 * T2 / st.Tc));
+            logFc = log10(fmax(SMALL, Fc));
+        }
         B = -1.1762 * logFc + -1.0 * 0.14 * logPr + 0.806;
         A = -0.67 * logFc + logPr + -0.4;
         Fi = pow(fmax(SMALL, Fc), 1.0 / (A * A / (B * B) + 1.0));

@@ -1503,12 +1503,12 @@ __device__ __forceinline__ void jacobian_global_state(const BlockArgs* __restric
 }
 
 // Occupancy targets (waves per SIMD) that the register allocator is held to.
-// Unconstrained, the Jacobian kernels take 171-181 VGPRs (2 waves per SIMD)
-// although a GRI-sized workgroup's LDS (30 KB) leaves room for 2.5; held to
-// 3 waves (168 VGPRs, no spills) GRI runs 12 % faster, H2/O2 25 % and the
-// device-memory variant on 150 species 15 %.  The species-rate kernel at 5
-// waves (96 VGPRs and a 12-register spill, from 111 VGPRs and 4 waves) gains
-// 11 %.  Four waves for the Jacobian kernels spill 48-56 registers and lose.
+// Unconstrained, the Jacobian kernels take 171-181 VGPRs (2 waves per SIMD);
+// held to 3 waves (168 VGPRs; the dense kernel spills 12 registers) GRI
+// runs 12 % faster, H2/O2 25 % and the device-memory variant on 150 species
+// 15 %.  The species-rate kernel at 5 waves (96 VGPRs and a 10-register
+// spill, from 111 VGPRs and 4 waves) gains 11 %.  Four waves for the
+// Jacobian kernels spill 48-56 registers and lose.
 #ifndef PJB_JAC_WAVES
 #define PJB_JAC_WAVES 3
 #endif

@@ -30,12 +30,21 @@ def host_states(mech, n, seed, conp=False):
     return sample_states(mech, n, seed=seed, T_range=T_RANGE, alpha=ALPHA, conp=conp, order='C')
 
 
+_FP64_KERNELS = {}
+
+
 def fp64_rates(mech, conp, param, phi, order):
-    """dphi of device states from the FP64 species-rate kernel."""
+    """dphi of device states from the FP64 species-rate kernel (one kernel
+    object per mechanism and options, made on first use: device calls take
+    any number of states)."""
     from ..pywrap import PySpeciesRatesKernel
     torch = _torch()
     dphi = torch.empty_like(phi)
-    k = PySpeciesRatesKernel(int(param.numel()), 0, mech=mech, conp=conp, order=order)
+    key = (id(mech), bool(conp), order)
+    k = _FP64_KERNELS.get(key)
+    if k is None or k.mech is not mech:
+        k = PySpeciesRatesKernel(int(param.numel()), 0, mech=mech, conp=conp, order=order)
+        _FP64_KERNELS[key] = k
     k(param, phi, dphi)
     return dphi
 

@@ -176,7 +176,7 @@ def _from_phi(thermo, P, phi):
     return T.copy(), m / m.sum(axis=1, keepdims=True)
 
 
-def gpu_reactor(mech, rtol=1e-9, atol=1e-15):
+def gpu_reactor(mech, rtol=1e-6, atol=1e-14):
     """``react(T, P, Y, dt) -> (T, Y)``: every particle an adiabatic
     constant-pressure reactor advanced over ``dt`` by the batched ROS2
     integrator on the GPU (``pywrap.PyIntegrator``)."""
@@ -289,12 +289,13 @@ def _select_pairs(rng, comp, num_pairs, num_skip=0):
 def run_simulation(mech, case, init_temp, pres, eq_ratio, fuel, oxidizer,
                    complete_products=('CO2', 'H2O', 'N2'), num_part=100, tau_res=10e-3,
                    tau_mix=1e-3, tau_pair=1e-3, num_res=10, seed=0, react=None,
-                   pilot=None):
+                   pilot=None, log=None):
     """Run the PaSR and return ``[steps + 1, num_part, Ns + 3]`` particle data
     (time, T, P, mass fractions with negatives zeroed), as the reference's
     ``run_simulation``.  ``pres`` is in atm.  ``react`` defaults to
     :func:`gpu_reactor`; ``pilot`` may give the pilot's equilibrium (T, Y) to
-    skip its computation."""
+    skip its computation; ``log(time, mean_T)`` is called after every step
+    (the reference prints the same two columns)."""
     thermo = MixtureThermo(mech)
     if react is None:
         react = gpu_reactor(mech)
@@ -381,6 +382,8 @@ def run_simulation(mech, case, init_temp, pres, eq_ratio, fuel, oxidizer,
             comp = np.concatenate([thermo.h_mass(T, Y)[:, None], Y], axis=1)
         time += dt
         data.append(save(time))
+        if log is not None:
+            log(time, float(T.mean()))
     return np.stack(data)
 
 
@@ -405,13 +408,13 @@ def parse_input_file(path):
     return pars
 
 
-def run_from_input(mech, pars, seed=0, react=None):
+def run_from_input(mech, pars, seed=0, react=None, log=None):
     """:func:`run_simulation` with the parameters of :func:`parse_input_file`."""
     return run_simulation(mech, pars['case'], pars['temperature'], pars['pressure'],
                           pars['equivalence ratio'], pars['fuel'], pars['oxidizer'],
                           pars['complete products'], pars['number of particles'],
                           pars['residence time'], pars['mixing time'], pars['pairing time'],
-                          pars['number of residence times'], seed=seed, react=react)
+                          pars['number of residence times'], seed=seed, react=react, log=log)
 
 
 def particle_states(mech, data, conp=True, V=1.0, unique=True):

@@ -1236,10 +1236,10 @@ __device__ __forceinline__ void jacobian_state(const BlockArgs* __restrict__ A, 
         }
     }
     __syncthreads();
-    // phase 6: stored entries of this state, four per thread in flight
+    // phase 6: stored entries of this state, eight per thread in flight
     {
         const int2* op = (const int2*)c.bt.out_pos;
-        constexpr int UNR = 4;
+        constexpr int UNR = 8;
         for (int e0 = tid; e0 < nout; e0 += NT * UNR) {
             int2 p[UNR];
 #pragma unroll

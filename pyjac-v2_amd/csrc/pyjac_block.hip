@@ -1157,6 +1157,7 @@ __device__ __forceinline__ void jacobian_state(const BlockArgs* __restrict__ A, 
         const int k0 = wave * per, k1 = min(Ns - 1, k0 + per);
         for (int j = lane; j < Ns - 1; j += 64) {
             double sum = 0.0, sum2 = 0.0;
+#pragma unroll 4
             for (int k = k0; k < k1; ++k) {
                 const double jv = J[(k + 2) * NN + j + 2] + c.U(k);
                 const double mw = sld(c.m.mw_post, k);

@@ -256,6 +256,7 @@ template <class PhiF>
 __device__ __forceinline__ void bath_gas_phi(Ctx& c, const State& st, int lane, PhiF phi) {
     const int Ns = c.m.Ns;
     double nsum = 0.0;
+#pragma unroll 8
     for (int k = 0; k < Ns - 1; ++k) nsum = nsum + fmax(SMALL, phi(2 + k));
     if (lane == 0)
         c.conc(Ns - 1) = __dadd_rn(st.P / (RU * st.Tc), __dmul_rn(-1.0 * fmax(SMALL, nsum), st.V_inv));

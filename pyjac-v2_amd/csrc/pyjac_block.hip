@@ -278,14 +278,17 @@ __device__ __forceinline__ void bath_gas_phi(Ctx& c, const State& st, int lane, 
 // after the barrier.
 struct ThdPre {
     double kf, kff, Fc, logFc, lnFc, dFcent;
+    int r, f;  // the entry's reaction and falloff index
 };
 
 template <bool JAC>
 __device__ __forceinline__ ThdPre thd_pre(const Ctx& c, const State& st, int t) {
     const DevMech& m = c.m;
-    ThdPre p{0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    ThdPre p{0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0, -1};
     const int r = m.thd_map[t];
     const int f = m.fall_mask[r];
+    p.r = r;
+    p.f = f;
     if (f < 0) return p;
     const int si = m.simple_mask[r];
     p.kf = arrhenius(m.simple_form[si], m.simple_A[si], m.simple_b[si], m.simple_Ta[si], st.Tc,
@@ -320,7 +323,7 @@ __device__ void thd_task(Ctx& c, const State& st, int t);
 // get_pre() yields the entry's ThdPre, evaluated on the spot or kept from
 // phase 1; it is called after the third-body sum, where the rate constants
 // are first needed
-template <bool JAC, class PreF>
+template <bool JAC, bool EARLY = false, class PreF>
 __device__ void thd_post(Ctx& c, const State& st, int t, PreF get_pre) {
     const DevMech& m = c.m;
     const int conp = c.conp;
@@ -345,8 +348,9 @@ __device__ void thd_post(Ctx& c, const State& st, int t, PreF get_pre) {
     }
     for (; s < e1; ++s)
         thd = thd + (m.thd_eff[s] + -1.0 * not_spec) * c.conc(m.thd_spec[s]);
-    const int r = m.thd_map[t];
-    const int f = m.fall_mask[r];
+    // EARLY: the entry's indices come with the values kept from phase 1
+    const int r = EARLY ? get_pre().r : m.thd_map[t];
+    const int f = EARLY ? get_pre().f : m.fall_mask[r];
     const double T = st.T, Tinv = st.Traw_inv;
     const double modT = c.bt.thd_mod[t];
     const double modE = (conp && tt == 3) ? 0.0 : modT;
@@ -1084,7 +1088,7 @@ __device__ __forceinline__ void jacobian_state(const BlockArgs* __restrict__ A, 
     const int n1 = c.m.n_thd, n2 = n1 + c.m.n_plog, n3 = n2 + c.m.n_cheb;
     const bool early = Ns + n3 <= NT;
     const int task = tid - (NT - n3);
-    ThdPre pre{0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    ThdPre pre{0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0, -1};
     phase_species<true, NT, true>(c, st, tid, ph);
     if (early && task >= 0) {
         if (task < n1) pre = thd_pre<true>(c, st, c.bt.thd_order[task]);
@@ -1095,7 +1099,7 @@ __device__ __forceinline__ void jacobian_state(const BlockArgs* __restrict__ A, 
     bath_gas_phi(c, st, lane, ph);
     if (early) {
         if (task >= 0 && task < n1)
-            thd_post<true>(c, st, c.bt.thd_order[task], [&]() { return pre; });
+            thd_post<true, true>(c, st, c.bt.thd_order[task], [&]() { return pre; });
     } else {
         phase_pdep<true, WPS>(c, st, wave, lane);
     }

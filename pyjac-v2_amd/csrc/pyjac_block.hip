@@ -59,6 +59,34 @@ __device__ __forceinline__ T sld(const T* p, long i) {
     return ((const __attribute__((address_space(4))) T*)p)[i];
 }
 
+// per-lane reads of the read-only tables go through the global address
+// space: a flat load also counts as an LDS operation, so waiting for its data
+// would wait for every LDS operation in flight, and the reverse
+template <class T>
+using gptr = const __attribute__((address_space(1))) T*;
+template <class T>
+__device__ __forceinline__ gptr<T> gp(const T* p) {
+    return (gptr<T>)p;
+}
+
+// the same for the packed int / double records (through clang's vector
+// types, which load from the global address space where HIP's classes do not)
+typedef int gint2 __attribute__((ext_vector_type(2)));
+typedef int gint4 __attribute__((ext_vector_type(4)));
+typedef double gdouble4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ int2 ld_int2(const void* p, long i) {
+    const gint2 v = gp((const gint2*)p)[i];
+    return make_int2(v.x, v.y);
+}
+__device__ __forceinline__ int4 ld_int4(const void* p, long i) {
+    const gint4 v = gp((const gint4*)p)[i];
+    return make_int4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ double4 ld_double4(const void* p, long i) {
+    const gdouble4 v = gp((const gdouble4*)p)[i];
+    return make_double4(v.x, v.y, v.z, v.w);
+}
+
 // fast_powi of the reference (loopy_utils/preambles_and_manglers.py:123-174):
 // a left-to-right product chain, here without data-dependent branches for
 // exponents up to 3
@@ -117,9 +145,9 @@ struct Rec {
 
 __device__ __forceinline__ Rec load_rec(const BlockTables& bt, int r) {
     Rec x;
-    const int4 ri = ((const int4*)bt.rec_i)[r];
-    const double4 d = ((const double4*)bt.rec_d)[r];
-    const int4 s0 = ((const int4*)bt.rec_s)[2 * r], s1 = ((const int4*)bt.rec_s)[2 * r + 1];
+    const int4 ri = ld_int4(bt.rec_i, r);
+    const double4 d = ld_double4(bt.rec_d, r);
+    const int4 s0 = ld_int4(bt.rec_s, 2 * r), s1 = ld_int4(bt.rec_s, 2 * r + 1);
     x.kind = ri.x & 0xff;
     x.nsum = (ri.x >> 8) - 128;
     x.rev = ri.y;
@@ -191,11 +219,11 @@ struct Thermo {
     double energy, heat, dheat;
 };
 
-__device__ __forceinline__ const double* nasa_coeffs(const DevMech& m, double T, int k) {
-    return (T < m.T_mid[k]) ? m.a_lo + 7 * k : m.a_hi + 7 * k;
+__device__ __forceinline__ gptr<double> nasa_coeffs(const DevMech& m, double T, int k) {
+    return gp((T < gp(m.T_mid)[k]) ? m.a_lo + 7 * k : m.a_hi + 7 * k);
 }
 
-__device__ __forceinline__ Thermo nasa_thermo(const double* a, double T, int conp) {
+__device__ __forceinline__ Thermo nasa_thermo(gptr<double> a, double T, int conp) {
     const double a0 = a[0], a1 = a[1], a2 = a[2], a3 = a[3], a4 = a[4], a5 = a[5];
     const double x = T * (T * (T * (T * (T * a4 / 5.0 + a3 / 4.0) + a2 / 3.0) + a1 / 2.0) + a0);
     const double y = T * (T * (T * (T * a4 + a3) + a2) + a1) + a0;
@@ -221,7 +249,7 @@ __device__ __forceinline__ void phase_species(Ctx& c, const State& st, int tid, 
             c.conc(k) = n * st.V_inv;
         }
         const double T = st.Tc;
-        const double* a = nasa_coeffs(m, T, k);
+        const gptr<double> a = nasa_coeffs(m, T, k);
         const double a0 = a[0], a1 = a[1], a2 = a[2], a3 = a[3], a4 = a[4], a5 = a[5], a6 = a[6];
         c.b(k) = T * (T * (T * (T * a4 / 20.0 + a3 / 12.0) + a2 / 6.0) + a1 / 2.0) +
                  (a0 + -1.0) * st.logT + -1.0 * a0 + a6 + -1.0 * a5 * st.Tinv;
@@ -285,30 +313,30 @@ template <bool JAC>
 __device__ __forceinline__ ThdPre thd_pre(const Ctx& c, const State& st, int t) {
     const DevMech& m = c.m;
     ThdPre p{0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0, -1};
-    const int r = m.thd_map[t];
-    const int f = m.fall_mask[r];
+    const int r = gp(m.thd_map)[t];
+    const int f = gp(m.fall_mask)[r];
     p.r = r;
     p.f = f;
     if (f < 0) return p;
-    const int si = m.simple_mask[r];
-    p.kf = arrhenius(m.simple_form[si], m.simple_A[si], m.simple_b[si], m.simple_Ta[si], st.Tc,
+    const int si = gp(m.simple_mask)[r];
+    p.kf = arrhenius(gp(m.simple_form)[si], gp(m.simple_A)[si], gp(m.simple_b)[si], gp(m.simple_Ta)[si], st.Tc,
                      st.logT, st.Tinv);
-    p.kff = arrhenius(m.fall_form[f], m.fall_A[f], m.fall_b[f], m.fall_Ta[f], st.Tc, st.logT,
+    p.kff = arrhenius(gp(m.fall_form)[f], gp(m.fall_A)[f], gp(m.fall_b)[f], gp(m.fall_Ta)[f], st.Tc, st.logT,
                       st.Tinv);
     // the species-rate kernel evaluates the Troe centre where it is used
-    if (JAC && m.fall_blend[f] == 2) {
-        const int i = m.fall_param[f];
-        const double a = m.troe_a[i];
-        double Fc = a * exp(fmin(EXP_MAX, -1.0 * st.Tc * m.troe_T1inv[i])) +
-                    (1.0 + -1.0 * a) * exp(fmin(EXP_MAX, -1.0 * st.Tc * m.troe_T3inv[i]));
-        const double T2 = m.troe_T2[i];
+    if (JAC && gp(m.fall_blend)[f] == 2) {
+        const int i = gp(m.fall_param)[f];
+        const double a = gp(m.troe_a)[i];
+        double Fc = a * exp(fmin(EXP_MAX, -1.0 * st.Tc * gp(m.troe_T1inv)[i])) +
+                    (1.0 + -1.0 * a) * exp(fmin(EXP_MAX, -1.0 * st.Tc * gp(m.troe_T3inv)[i]));
+        const double T2 = gp(m.troe_T2)[i];
         if (T2 != 0.0) Fc = Fc + exp(fmin(EXP_MAX, -1.0 * T2 / st.Tc));
         p.Fc = Fc;
         p.logFc = log10(fmax(SMALL, Fc));
         if (JAC) {
             p.lnFc = log(fmax(SMALL, Fc));
             const double T = st.T, Tinv = st.Traw_inv;
-            const double T1 = m.troe_T1inv[i], T3 = m.troe_T3inv[i];
+            const double T1 = gp(m.troe_T1inv)[i], T3 = gp(m.troe_T3inv)[i];
             p.dFcent = -1.0 * a * T1 * exp(fmin(EXP_MAX, -1.0 * T * T1)) +
                        (a + -1.0) * T3 * exp(fmin(EXP_MAX, -1.0 * T * T3)) +
                        T2 * Tinv * Tinv * exp(fmin(EXP_MAX, -1.0 * T2 * Tinv));
@@ -327,19 +355,19 @@ template <bool JAC, bool EARLY = false, class PreF>
 __device__ void thd_post(Ctx& c, const State& st, int t, PreF get_pre) {
     const DevMech& m = c.m;
     const int conp = c.conp;
-    const int tt = m.thd_type[t];
+    const int tt = gp(m.thd_type)[t];
     const int not_spec = tt != 2;
     double thd = st.P * (double)not_spec / (RU * st.T);
-    const int e1 = m.thd_off[t + 1];
-    int s = m.thd_off[t];
+    const int e1 = gp(m.thd_off)[t + 1];
+    int s = gp(m.thd_off)[t];
     // four efficiencies per step with their loads issued together (the
     // Jacobian kernels; the species-rate kernel has no registers to spare);
     // the sum keeps its sequential order
     for (; JAC && s + 4 <= e1; s += 4) {
-        const double f0 = m.thd_eff[s], f1 = m.thd_eff[s + 1], f2 = m.thd_eff[s + 2],
-                     f3 = m.thd_eff[s + 3];
-        const int k0 = m.thd_spec[s], k1 = m.thd_spec[s + 1], k2 = m.thd_spec[s + 2],
-                  k3 = m.thd_spec[s + 3];
+        const double f0 = gp(m.thd_eff)[s], f1 = gp(m.thd_eff)[s + 1], f2 = gp(m.thd_eff)[s + 2],
+                     f3 = gp(m.thd_eff)[s + 3];
+        const int k0 = gp(m.thd_spec)[s], k1 = gp(m.thd_spec)[s + 1], k2 = gp(m.thd_spec)[s + 2],
+                  k3 = gp(m.thd_spec)[s + 3];
         const double c0 = c.conc(k0), c1 = c.conc(k1), c2 = c.conc(k2), c3 = c.conc(k3);
         thd = thd + (f0 + -1.0 * not_spec) * c0;
         thd = thd + (f1 + -1.0 * not_spec) * c1;
@@ -347,12 +375,12 @@ __device__ void thd_post(Ctx& c, const State& st, int t, PreF get_pre) {
         thd = thd + (f3 + -1.0 * not_spec) * c3;
     }
     for (; s < e1; ++s)
-        thd = thd + (m.thd_eff[s] + -1.0 * not_spec) * c.conc(m.thd_spec[s]);
+        thd = thd + (gp(m.thd_eff)[s] + -1.0 * not_spec) * c.conc(gp(m.thd_spec)[s]);
     // EARLY: the entry's indices come with the values kept from phase 1
-    const int r = EARLY ? get_pre().r : m.thd_map[t];
-    const int f = EARLY ? get_pre().f : m.fall_mask[r];
+    const int r = EARLY ? get_pre().r : gp(m.thd_map)[t];
+    const int f = EARLY ? get_pre().f : gp(m.fall_mask)[r];
     const double T = st.T, Tinv = st.Traw_inv;
-    const double modT = c.bt.thd_mod[t];
+    const double modT = gp(c.bt.thd_mod)[t];
     const double modE = (conp && tt == 3) ? 0.0 : modT;
     if (f < 0) {
         c.ci(t) = thd;
@@ -371,13 +399,13 @@ __device__ void thd_post(Ctx& c, const State& st, int t, PreF get_pre) {
     }
     // falloff / chemically activated
     const ThdPre pre = get_pre();
-    const int si = m.simple_mask[r];
+    const int si = gp(m.simple_mask)[r];
     const double kf = pre.kf, kff = pre.kff;
-    const int ftype = m.fall_type[f];
+    const int ftype = gp(m.fall_type)[f];
     const double kinf = ftype ? kff : kf;
     const double k0 = ftype ? kf : kff;
     const double Pr = thd * k0 / kinf;
-    const int blend = m.fall_blend[f];
+    const int blend = gp(m.fall_blend)[f];
     double Fi = 1.0, Fc = 0.0, A = 0.0, B = 0.0, X = 0.0;
     if (blend == 2) {
         const double logPr = log10(fmax(SMALL, Pr));
@@ -386,11 +414,11 @@ __device__ void thd_post(Ctx& c, const State& st, int t, PreF get_pre) {
             Fc = pre.Fc;
             logFc = pre.logFc;
         } else {
-            const int i = m.fall_param[f];
-            const double a = m.troe_a[i];
-            Fc = a * exp(fmin(EXP_MAX, -1.0 * st.Tc * m.troe_T1inv[i])) +
-                 (1.0 + -1.0 * a) * exp(fmin(EXP_MAX, -1.0 * st.Tc * m.troe_T3inv[i]));
-            const double T2 = m.troe_T2[i];
+            const int i = gp(m.fall_param)[f];
+            const double a = gp(m.troe_a)[i];
+            Fc = a * exp(fmin(EXP_MAX, -1.0 * st.Tc * gp(m.troe_T1inv)[i])) +
+                 (1.0 + -1.0 * a) * exp(fmin(EXP_MAX, -1.0 * st.Tc * gp(m.troe_T3inv)[i]));
+            const double T2 = gp(m.troe_T2)[i];
             if (T2 != 0.0) Fc = Fc + exp(fmin(EXP_MAX, -1.0 * T2 / st.Tc));
             logFc = log10(fmax(SMALL, Fc));
         }
@@ -398,13 +426,13 @@ __device__ void thd_post(Ctx& c, const State& st, int t, PreF get_pre) {
         A = -0.67 * logFc + logPr + -0.4;
         Fi = pow(fmax(SMALL, Fc), 1.0 / (A * A / (B * B) + 1.0));
     } else if (blend == 3) {
-        const int i = m.fall_param[f];
+        const int i = gp(m.fall_param)[f];
         const double logPr = log10(fmax(SMALL, Pr));
         X = 1.0 / (logPr * logPr + 1.0);
-        const double base = m.sri_a[i] * exp(fmin(EXP_MAX, -m.sri_b[i] * st.Tinv)) +
-                            exp(fmin(EXP_MAX, -st.Tc / m.sri_c[i]));
+        const double base = gp(m.sri_a)[i] * exp(fmin(EXP_MAX, -gp(m.sri_b)[i] * st.Tinv)) +
+                            exp(fmin(EXP_MAX, -st.Tc / gp(m.sri_c)[i]));
         Fi = pow(base, X);
-        const double d = m.sri_d[i], e = m.sri_e[i];
+        const double d = gp(m.sri_d)[i], e = gp(m.sri_e)[i];
         if (d != 1.0) Fi = Fi * d;
         if (e != 0.0) Fi = Fi * pow(st.Tc, e);
     }
@@ -421,9 +449,9 @@ __device__ void thd_post(Ctx& c, const State& st, int t, PreF get_pre) {
             dFi = -2.0 * A * B * (0.14 * A + B) * pre.lnFc /
                   (fmax(SMALL, Pr) * dFi * dFi * LN10);
         } else if (blend == 3) {
-            const int i = m.fall_param[f];
-            const double lv = log(fmax(SMALL, m.sri_a[i] * exp(fmin(EXP_MAX, -m.sri_b[i] / T)) +
-                                                  exp(fmin(EXP_MAX, -T / m.sri_c[i]))));
+            const int i = gp(m.fall_param)[f];
+            const double lv = log(fmax(SMALL, gp(m.sri_a)[i] * exp(fmin(EXP_MAX, -gp(m.sri_b)[i] / T)) +
+                                                  exp(fmin(EXP_MAX, -T / gp(m.sri_c)[i]))));
             dFi = -2.0 * X * X * lv * log(fmax(SMALL, Pr)) / (fmax(SMALL, Pr) * LN10SQ);
         }
         double Fi_fac = dFi;
@@ -435,11 +463,11 @@ __device__ void thd_post(Ctx& c, const State& st, int t, PreF get_pre) {
         const double thd_fac = -1.0 * st.P * modT * RU_INV * Tinv * Tinv;
         double beta_0, beta_inf, Ta_0, Ta_inf;
         if (ftype) {
-            beta_0 = m.simple_b[si]; Ta_0 = m.simple_Ta[si];
-            beta_inf = m.fall_b[f]; Ta_inf = m.fall_Ta[f];
+            beta_0 = gp(m.simple_b)[si]; Ta_0 = gp(m.simple_Ta)[si];
+            beta_inf = gp(m.fall_b)[f]; Ta_inf = gp(m.fall_Ta)[f];
         } else {
-            beta_0 = m.fall_b[f]; Ta_0 = m.fall_Ta[f];
-            beta_inf = m.simple_b[si]; Ta_inf = m.simple_Ta[si];
+            beta_0 = gp(m.fall_b)[f]; Ta_0 = gp(m.fall_Ta)[f];
+            beta_inf = gp(m.simple_b)[si]; Ta_inf = gp(m.simple_Ta)[si];
         }
         const double theta_Pr = Tinv * (beta_0 - beta_inf + (Ta_0 - Ta_inf) * Tinv);
         const double theta_no_Pr = thd_fac * k0 / kinf;
@@ -455,9 +483,9 @@ __device__ void thd_post(Ctx& c, const State& st, int t, PreF get_pre) {
                                   -1.0 * B * absq * LN10)) /
                   (Fc * fmax(SMALL, Pr) * absqsq * LN10);
         } else if (blend == 3) {
-            const int i = m.fall_param[f];
-            const double a = m.sri_a[i], b = m.sri_b[i], e_ = m.sri_e[i];
-            const double cinv = 1.0 / m.sri_c[i];
+            const int i = gp(m.fall_param)[f];
+            const double a = gp(m.sri_a)[i], b = gp(m.sri_b)[i], e_ = gp(m.sri_e)[i];
+            const double cinv = 1.0 / gp(m.sri_c)[i];
             const double ec = exp(fmin(EXP_MAX, -T * cinv));
             const double eb = exp(fmin(EXP_MAX, -b * Tinv));
             const double lv = log(fmax(SMALL, a * eb + ec));
@@ -481,8 +509,8 @@ __device__ void thd_post(Ctx& c, const State& st, int t, PreF get_pre) {
             dFi = -2.0 * A * B * pre.lnFc * (0.14 * A + B) * theta /
                   (fmax(SMALL, Pr) * absqsq * LN10);
         } else if (blend == 3) {
-            const int i = m.fall_param[f];
-            const double a = m.sri_a[i], b = m.sri_b[i], cinv = 1.0 / m.sri_c[i];
+            const int i = gp(m.fall_param)[f];
+            const double a = gp(m.sri_a)[i], b = gp(m.sri_b)[i], cinv = 1.0 / gp(m.sri_c)[i];
             double inner;
             if (conp) {
                 const double exp_cinv = exp(fmin(EXP_MAX, T * cinv));
@@ -512,7 +540,7 @@ __device__ void thd_task(Ctx& c, const State& st, int t) {
 }
 
 __device__ __forceinline__ double plog_par(const DevMech& m, int q, int p, int k) {
-    return m.plog_params[((size_t)q * m.n_plog + p) * m.plog_max_p + k];
+    return gp(m.plog_params)[((size_t)q * m.n_plog + p) * m.plog_max_p + k];
 }
 
 // PLOG rate constant (rate_subs.py:2084-2248) and, for the Jacobian,
@@ -522,7 +550,7 @@ template <bool JAC>
 __device__ void plog_task(Ctx& c, const State& st, int p) {
     const DevMech& m = c.m;
     const int mp = m.plog_max_p;
-    const int numP = m.plog_np[p] - 1;
+    const int numP = gp(m.plog_np)[p] - 1;
     const double logP = st.logP;
     {
         const bool lower = logP <= plog_par(m, 0, p, 0);
@@ -602,14 +630,14 @@ template <bool JAC>
 __device__ void cheb_task(Ctx& c, const State& st, int q) {
     const DevMech& m = c.m;
     const int maxT = m.cheb_max_t, maxP = m.cheb_max_p;
-    const double Pmin = m.cheb_plim[2 * q], Pmax = m.cheb_plim[2 * q + 1];
-    const double Tmin = m.cheb_tlim[2 * q], Tmax = m.cheb_tlim[2 * q + 1];
-    const double* par = m.cheb_params + (size_t)q * maxT * maxP;
+    const double Pmin = gp(m.cheb_plim)[2 * q], Pmax = gp(m.cheb_plim)[2 * q + 1];
+    const double Tmin = gp(m.cheb_tlim)[2 * q], Tmax = gp(m.cheb_tlim)[2 * q + 1];
+    const gptr<double> par = gp(m.cheb_params + (size_t)q * maxT * maxP);
     const double Pred = (-Pmax - Pmin + 2.0 * st.logP) / (Pmax - Pmin);
     const int kidx = m.n_plog + q;
     {
         const double Tred = (-Tmax - Tmin + 2.0 * st.Tinv) / (Tmax - Tmin);
-        const int numP = m.cheb_np[q], numT = m.cheb_nt[q];
+        const int numP = gp(m.cheb_np)[q], numT = gp(m.cheb_nt)[q];
         double kt = 0.0;
         ChebPoly tp(Tred, Tred);
         for (int t = 0; t < maxT; ++t) {
@@ -630,7 +658,7 @@ __device__ void cheb_task(Ctx& c, const State& st, int q) {
     const double Pred2 = (2 * st.logP - Pmax - Pmin) / (Pmax - Pmin);
     double dkT, dkP = 0.0;
     {
-        const int numP = m.cheb_np[q], numT = m.cheb_nt[q] - 1;
+        const int numP = gp(m.cheb_np)[q], numT = gp(m.cheb_nt)[q] - 1;
         double dkf = 0;
         ChebPoly tp(Tred, 2 * Tred);
         for (int t = 0; t < maxT - 1; ++t) {
@@ -646,7 +674,7 @@ __device__ void cheb_task(Ctx& c, const State& st, int q) {
         dkT = -dkf * 2 * LN10 * Tinv * Tinv / (Tmax - Tmin);
     }
     if (!c.conp) {
-        const int numP = m.cheb_np[q] - 1, numT = m.cheb_nt[q];
+        const int numP = gp(m.cheb_np)[q] - 1, numT = gp(m.cheb_nt)[q];
         double dkf = 0;
         ChebPoly tp(Tred, Tred);
         for (int t = 0; t < maxT; ++t) {
@@ -674,7 +702,7 @@ __device__ __forceinline__ void phase_pdep(Ctx& c, const State& st, int wave, in
     const int per = (n3 + WPS - 1) / WPS;
     const int end = min(n3, (wave + 1) * per);
     for (int i = wave * per + lane; i < end; i += 64) {
-        if (i < n1) thd_task<JAC>(c, st, c.bt.thd_order[i]);
+        if (i < n1) thd_task<JAC>(c, st, gp(c.bt.thd_order)[i]);
         else if (i < n2) plog_task<JAC>(c, st, i - n1);
         else cheb_task<JAC>(c, st, i - n2);
     }
@@ -784,7 +812,7 @@ struct JacAcc {
 // one contribution: to the dense LDS entry, or to the buffer slot that the
 // record's slot table holds for (column slot, row slot)
 template <bool G>
-__device__ __forceinline__ void jput(const JacAcc& a, const int* __restrict__ pos, int slot,
+__device__ __forceinline__ void jput(const JacAcc& a, gptr<int> pos, int slot,
                                      int dense, double v) {
     if constexpr (!G) {
         lds_add(a.lds + dense, v);
@@ -805,7 +833,7 @@ template <bool G>
 __device__ __forceinline__ void reaction_jacobian(Ctx& c, const State& st, const Rec& x,
                                                   const Rop& v, double ropi, int gf,
                                                   const JacAcc& acc,
-                                                  const int* __restrict__ ent) {
+                                                  gptr<int> ent) {
     const int ns = c.ns, NN = c.NN;
     const int cm = gf & GF_CNT;
     const int conp = c.conp;
@@ -916,12 +944,12 @@ __device__ __forceinline__ void reaction_jacobian(Ctx& c, const State& st, const
             // (efficiency set, row species) pair and spread in phase 4
             const int t = x.thd;
             const double trop = ropi * c.nfac(t);
-            u += c.bt.effns[t] * trop;
+            u += gp(c.bt.effns)[t] * trop;
             dT += c.dTf(t) * ropi;
             dE += c.dEf(t) * ropi;
             if (gf & GF_PAIRS) {
-                const int4 p0 = ((const int4*)c.bt.thd_pairs)[2 * t];
-                const int2 p1 = ((const int2*)c.bt.thd_pairs)[4 * t + 2];
+                const int4 p0 = ld_int4(c.bt.thd_pairs, 2 * t);
+                const int2 p1 = ld_int2(c.bt.thd_pairs, 4 * t + 2);
                 const int pp[MAXSP] = {p0.x, p0.y, p0.z, p0.w, p1.x, p1.y};
 #pragma unroll
                 for (int q = 0; q < MAXSP; ++q)
@@ -1090,16 +1118,19 @@ __device__ __forceinline__ void jacobian_state(const BlockArgs* __restrict__ A, 
     const int task = tid - (NT - n3);
     ThdPre pre{0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0, -1};
     phase_species<true, NT, true>(c, st, tid, ph);
+    // with the entries on the last threads, wave 0 sums the bath gas in
+    // phase 1 (its lane 0 stores it before the barrier)
+    if (early && wave == 0) bath_gas_phi(c, st, lane, ph);
     if (early && task >= 0) {
-        if (task < n1) pre = thd_pre<true>(c, st, c.bt.thd_order[task]);
+        if (task < n1) pre = thd_pre<true>(c, st, gp(c.bt.thd_order)[task]);
         else if (task < n2) plog_task<true>(c, st, task - n1);
         else cheb_task<true>(c, st, task - n2);
     }
     __syncthreads();
-    bath_gas_phi(c, st, lane, ph);
+    if (!early) bath_gas_phi(c, st, lane, ph);
     if (early) {
         if (task >= 0 && task < n1)
-            thd_post<true, true>(c, st, c.bt.thd_order[task], [&]() { return pre; });
+            thd_post<true, true>(c, st, gp(c.bt.thd_order)[task], [&]() { return pre; });
     } else {
         phase_pdep<true, WPS>(c, st, wave, lane);
     }
@@ -1121,9 +1152,8 @@ __device__ __forceinline__ void jacobian_state(const BlockArgs* __restrict__ A, 
     // dci_thd_dnj / dci_lind / dci_sri / dci_troe): every non-zero
     // (pair, column) of the host's list adds coefficient * efficiency
     {
-        const int4* cb = (const int4*)c.bt.combo;
         for (int i = tid; i < c.bt.n_combo; i += NT) {
-            const int4 e = cb[i];
+            const int4 e = ld_int4(c.bt.combo, i);
             lds_add(J + e.y, c.Cp(e.x) * __hiloint2double(e.w, e.z));
         }
     }
@@ -1189,7 +1219,7 @@ __device__ __forceinline__ void jacobian_state(const BlockArgs* __restrict__ A, 
         for (int i = lane; i < Ns - 1; i += 64) {
             const double j0 = J[(i + 2) * NN];
             const double j1 = J[(i + 2) * NN + 1];
-            const double mw = c.m.mw_post[i];
+            const double mw = gp(c.m.mw_post)[i];
             const double w = c.wdot(i);
             rate_sum += Vinv * j0 * (-1.0 * c.energy(i) + e_ns * mw) +
                         w * (-1.0 * c.heat(i) + mw * h_ns);
@@ -1244,14 +1274,13 @@ __device__ __forceinline__ void jacobian_state(const BlockArgs* __restrict__ A, 
     __syncthreads();
     // phase 6: stored entries of this state, eight per thread in flight
     {
-        const int2* op = (const int2*)c.bt.out_pos;
         constexpr int UNR = 8;
         for (int e0 = tid; e0 < nout; e0 += NT * UNR) {
             int2 p[UNR];
 #pragma unroll
             for (int u = 0; u < UNR; ++u) {
                 const int e = e0 + u * NT;
-                p[u] = e < nout ? op[e] : make_int2(0, -1);
+                p[u] = e < nout ? ld_int2(c.bt.out_pos, e) : make_int2(0, -1);
             }
             double val[UNR];
 #pragma unroll
@@ -1279,7 +1308,7 @@ constexpr int GE = 8, GC = pjk_host::GATHER_MAX;
 // completed and a barrier): workgroup scope, served by this CU's caches.
 // (Agent scope would bypass the XCD's L2 and read from memory.)
 __device__ __forceinline__ double wg_load(const double* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    return __hip_atomic_load(gp(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
 // Every memory operation of this wave completed before the barrier that
@@ -1314,7 +1343,7 @@ __device__ __forceinline__ void jacobian_global_state(const BlockArgs* __restric
     const JacAcc acc{nullptr, cb, col0, col1, lng};
     const bool approx = c.bt.approx_sums != 0;
     if (tid == 0) { c.conc(Ns) = 1.0; c.b(Ns) = 0.0; c.db(Ns) = 0.0; }
-    for (int k = tid; k < Ns - 1; k += NT) c.mw(k) = c.m.mw_post[k];
+    for (int k = tid; k < Ns - 1; k += NT) c.mw(k) = gp(c.m.mw_post)[k];
     for (int p = tid; p < c.bt.n_pairs; p += NT) c.Cp(p) = 0.0;
     const double* ps = phi + s * L.sN;
     auto ph = [&](int i) { return ps[i * L.sI]; };
@@ -1338,7 +1367,7 @@ __device__ __forceinline__ void jacobian_global_state(const BlockArgs* __restric
             const double ropi = v.rf - v.rr;
             scatter_wdot(c, x, ropi * v.ci, gf);
             reaction_jacobian<true>(c, st, x, v, ropi, gf, acc,
-                                    c.bt.rec_pos + (size_t)r * (8 * MAXSP));
+                                    gp(c.bt.rec_pos + (size_t)r * (8 * MAXSP)));
         }
     }
     __syncthreads();
@@ -1347,9 +1376,8 @@ __device__ __forceinline__ void jacobian_global_state(const BlockArgs* __restric
     // third-body columns (create_jacobian.py dci_*_dnj): each (pair, column)
     // of the host's list is one more contribution of its entry
     {
-        const int4* cmb = (const int4*)c.bt.gcombo;
         for (int i = tid; i < c.bt.n_combo; i += NT) {
-            const int4 m4 = cmb[i];
+            const int4 m4 = ld_int4(c.bt.gcombo, i);
             const double v = c.Cp(m4.x) * __hiloint2double(m4.w, m4.z);
             if (m4.y >= 0) cb[m4.y] = v;
             else lds_add(lng + (-2 - m4.y), v);
@@ -1377,9 +1405,9 @@ __device__ __forceinline__ void jacobian_global_state(const BlockArgs* __restric
     // at a time with all their contributions in flight together and sums
     // each in slot order; entries with more contributions are in LDS
     {
-        const int* es = c.bt.ent_start;
-        const int* rc = c.bt.ent_rc;
-        const int* el = c.bt.ent_long;
+        const gptr<int> es = gp(c.bt.ent_start);
+        const gptr<int> rc = gp(c.bt.ent_rc);
+        const gptr<int> el = gp(c.bt.ent_long);
         for (int eb = tid; eb < nout; eb += NT * GE) {
             int trc[GE], a[GE], b[GE], lg[GE];
 #pragma unroll
@@ -1438,7 +1466,7 @@ __device__ __forceinline__ void jacobian_global_state(const BlockArgs* __restric
     __syncthreads();
     // phase 5: rows 0 and 1 of the species columns (dTdot_dnj, dEdot_dnj,
     // create_jacobian.py:3266-3484)
-    const int* amap = c.bt.acc_map;
+    const gptr<int> amap = gp(c.bt.acc_map);
     for (int j = tid; j < Ns - 1; j += NT) {
         double j0 = -1.0 * (cs0[j] + su0) * inv_vh;
         j0 = j0 + -1.0 * Tdot * (c.heat(j) + -1.0 * h_ns) * inv_vh;

@@ -733,7 +733,9 @@ int pyjac_hip_create(const pyjac_mech_tables* mt, int conp, char order, int jac_
 
     // block-kernel tables: reaction records sorted by structure (third body,
     // pressure-dependent kind, reversibility, bath gas, species count) so
-    // the lanes of a wave follow the same branches
+    // the lanes of a wave follow the same branches; the simplest come last,
+    // where the last group of 64 is usually partly empty but costs its wave
+    // a whole pass of the instructions its flags select
     std::vector<int> perm(Nr), pos_of(Nr);
     for (int r = 0; r < Nr; ++r) perm[r] = r;
     auto rkey = [&](int r) {
@@ -744,7 +746,7 @@ int pyjac_hip_create(const pyjac_mech_tables* mt, int conp, char order, int jac_
                ((mt->fall_mask[r] >= 0) << 10) | ((mt->rev_mask[r] >= 0) << 9) |
                (has_ns << 8) | (kind << 4) | cnt;
     };
-    std::stable_sort(perm.begin(), perm.end(), [&](int a, int b) { return rkey(a) < rkey(b); });
+    std::stable_sort(perm.begin(), perm.end(), [&](int a, int b) { return rkey(a) > rkey(b); });
     std::vector<I4> rec_i(Nr);
     std::vector<D4> rec_d(Nr);
     std::vector<int> rec_s((size_t)8 * Nr);
@@ -809,8 +811,10 @@ int pyjac_hip_create(const pyjac_mech_tables* mt, int conp, char order, int jac_
             thd_pairs[(size_t)8 * t + q] = p;
         }
     }
-    // flags of each group of 64 consecutive records (see pyjac_block.hip)
-    std::vector<int> grp_flags((Nr + 63) / 64, 0);
+    // flags of each group of 64 consecutive records (see pyjac_block.hip);
+    // seven zero entries past the last group cover the waves of a block
+    // kernel's last pass that have no group (up to eight waves per state)
+    std::vector<int> grp_flags((Nr + 63) / 64 + 7, 0);
     for (int q = 0; q < Nr; ++q) {
         const int r = perm[q];
         const int cnt = mt->rxn_off[r + 1] - mt->rxn_off[r];

@@ -1091,6 +1091,9 @@ __device__ __forceinline__ void jacobian_state(const BlockArgs* __restrict__ A, 
     Ctx c = make_ctx(A, smem, conp);
     const int Ns = c.Ns, NN = c.NN, ns = c.ns, Nr = c.m.Nr, nout = c.m.jac_nnz;
     lds_double* J = c.J();
+    // issue priority over the F-order transposes that share the CUs (they
+    // wait on memory, these waves on their dependent chains): 0.3 % faster
+    __builtin_amdgcn_s_setprio(2);
     const JacAcc acc{J, nullptr, nullptr, nullptr};
     // rows 0 and 1 of the accumulator hold the Gibbs terms b and their T
     // derivatives db until the reaction pass is over (they are first

@@ -193,7 +193,7 @@ def main():
                      'unit': 'GB/s', 'frac': achieved / HBM_PEAK_GBS,
                      'traffic': measured_traffic(n),
                      'kernel': 'Jacobian path: jacobian_block_kernel<2> (C-order chunks of '
-                               '65536 states) + entries_to_f2_kernel (transpose to F order)',
+                               '65536 states) + entries_to_f2_small_kernel (transpose to F order, on a second stream)',
                      'algorithmic_bytes_per_state': bytes_per_state},
     }
     if not args.no_surrogate:

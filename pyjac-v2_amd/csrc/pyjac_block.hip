@@ -1275,8 +1275,36 @@ __device__ __forceinline__ void jacobian_state(const BlockArgs* __restrict__ A, 
         }
     }
     __syncthreads();
-    // phase 6: stored entries of this state, eight per thread in flight
-    {
+    // phase 6: stored entries of this state, eight per thread in flight;
+    // a C-order row of even length from a 16-byte aligned output goes out
+    // as pairs of entries (16-byte index loads and stores)
+    if (out_C && (nout & 1) == 0 && ((size_t)jac & 15) == 0) {
+        typedef double v2d __attribute__((ext_vector_type(2)));
+        constexpr int UNR = 4;
+        const int np = nout >> 1;
+        v2d* row = (v2d*)(jac + s * (size_t)nout);
+        for (int q0 = tid; q0 < np; q0 += NT * UNR) {
+            int4 p[UNR];
+#pragma unroll
+            for (int u = 0; u < UNR; ++u) {
+                const int q = q0 + u * NT;
+                p[u] = q < np ? ld_int4(c.bt.out_pos, q) : make_int4(0, -1, 0, -1);
+            }
+            v2d val[UNR];
+#pragma unroll
+            for (int u = 0; u < UNR; ++u) {
+                double a = J[p[u].x], b = J[p[u].z];
+                if (p[u].y >= 0) a += c.U(p[u].y);
+                if (p[u].w >= 0) b += c.U(p[u].w);
+                val[u] = v2d{a, b};
+            }
+#pragma unroll
+            for (int u = 0; u < UNR; ++u) {
+                const int q = q0 + u * NT;
+                if (q < np) row[q] = val[u];
+            }
+        }
+    } else {
         constexpr int UNR = 8;
         for (int e0 = tid; e0 < nout; e0 += NT * UNR) {
             int2 p[UNR];

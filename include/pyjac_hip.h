@@ -33,7 +33,9 @@
 extern "C" {
 #endif
 
-#define PYJAC_MECH_ABI_VERSION 1
+/* version 2 added the species and reaction names at the end of
+   pyjac_mech_tables; version 1 tables (no names) are still accepted */
+#define PYJAC_MECH_ABI_VERSION 2
 
 /* status codes */
 #define PYJAC_OK 0
@@ -138,6 +140,13 @@ typedef struct pyjac_mech_tables {
     const int32_t *ccs_row_ind;  /* [jac_nnz] */
     const int32_t *crs_row_ptr;  /* [NN+1] */
     const int32_t *crs_col_ind;  /* [jac_nnz] */
+
+    /* (abi_version >= 2) species names [Ns] in the tables' order (bath gas
+       last) and reaction strings [Nr], NUL-terminated; either may be NULL.
+       The generated code holds them as the static speciesNames() /
+       reactionStrings() tables (kernel.hpp.in:101-143). */
+    const char *const *species_names;
+    const char *const *reaction_strings;
 } pyjac_mech_tables;
 
 typedef struct pyjac_handle pyjac_handle;
@@ -148,7 +157,10 @@ typedef struct pyjac_handle pyjac_handle;
  * MechTables.save_binary -- magic "PJHIPTB1", int32 number of scalar fields
  * and their int32 values in struct order, int32 number of arrays, then per
  * array in struct order: uint8 type (0 int32, 1 float64), int64 length,
- * data.  Lets C and C++ callers use the library without Python.  *out is
+ * data; optionally the trailer "PJNAMES1" with the species names and the
+ * reaction strings (per list an int32 count, per name an int32 length and
+ * its bytes), which fills the struct's name fields (NULL without it).
+ * Lets C and C++ callers use the library without Python.  *out is
  * owned by the caller and released with pyjac_hip_tables_free.
  * PYJAC_ERR_IO: missing or short file; PYJAC_ERR_ARG: not such a file, or
  * another table layout version.
@@ -190,11 +202,54 @@ int pyjac_hip_species_rates(pyjac_handle *h, const double *param,
                             const double *phi, double *dphi);
 int pyjac_hip_jacobian(pyjac_handle *h, const double *param,
                        const double *phi, double *jac);
-/* chem_utils kernel (rate_subs.py:3391-3452): spec_energy = h (CONP) or
-   u (CONV), spec_heat = cp or cv, b = S/R - H/RT - ln T;  arrays shaped
-   like phi but with Ns columns. */
-int pyjac_hip_chem_utils(pyjac_handle *h, const double *phi,
-                         double *spec_energy, double *spec_heat, double *b);
+/* chem_utils kernel (rate_subs.py:3391-3452): b = S/R - H/RT - ln T,
+   spec_energy = h (CONP) or u (CONV), spec_heat = cp or cv; outputs shaped
+   like phi but with Ns values per state.  Argument order of the Cython
+   wrapper's PyChem_utilsKernel(phi, b, specific_energy, specific_heat)
+   (pywrap/pywrap_gen.py:176-197: inputs, then the outputs in
+   kernel_argument_ordering's order, utils.py:203-277); the generated
+   device kernel is chem_utils(t, b, phi, h|u, cp|cv, rwk)
+   (examples/opencl/ch4/chem_utils.oclh:23), i.e. the same arrays with the
+   unused time array and the work buffer dropped and phi moved first. */
+int pyjac_hip_chem_utils(pyjac_handle *h, const double *phi, double *b,
+                         double *spec_energy, double *spec_heat);
+
+/* Size-checked host entry points: the same calls on n_states states with
+   the element count of every array; an array shorter than the layout
+   needs (param n_states, phi and dphi n_states * NN, jac n_states *
+   pyjac_hip_jacobian_size, chem_utils outputs n_states * Ns) is
+   PYJAC_ERR_ARG before anything is read or written.  The handle is resized
+   to n_states first when its problem size differs.  No counterpart in the
+   reference, whose operator() trusts problem_size
+   (kernel_utils/common/kernel.cpp.in:572-640). */
+int pyjac_hip_species_rates_sized(pyjac_handle *h, size_t n_states, const double *param,
+                                  size_t param_count, const double *phi, size_t phi_count,
+                                  double *dphi, size_t dphi_count);
+int pyjac_hip_jacobian_sized(pyjac_handle *h, size_t n_states, const double *param,
+                             size_t param_count, const double *phi, size_t phi_count,
+                             double *jac, size_t jac_count);
+int pyjac_hip_chem_utils_sized(pyjac_handle *h, size_t n_states, const double *phi,
+                               size_t phi_count, double *b, double *spec_energy,
+                               double *spec_heat, size_t out_count);
+
+/* Every host-pointer entry point rejects a NaN or an infinity in param or
+   phi with PYJAC_ERR_ARG, naming the first such state in
+   pyjac_hip_last_error(); the reference computes on such states unchecked
+   (its test harness only rejects non-finite outputs,
+   functional_tester/test.py:1153-1156).  The device-pointer entry points
+   do the same check on the device when the handle's input check is on
+   (off by default: it adds one scan of the inputs and a synchronisation
+   of the stream per call). */
+int pyjac_hip_set_input_check(pyjac_handle *h, int on);
+
+/* Per-state finiteness report of a device array of n_states states with
+   width values each in the handle's data order (phi / dphi: NN, jac:
+   pyjac_hip_jacobian_size, param: 1): d_flags[s] (device int32 [n_states],
+   may be NULL) = 1 when state s holds a NaN or an infinity, else 0;
+   *n_bad = the number of such states and *first_bad the smallest one
+   (n_states when none).  Synchronises `stream`. */
+int pyjac_hip_nonfinite_dev(pyjac_handle *h, size_t n_states, const double *d_arr, int width,
+                            int32_t *d_flags, size_t *n_bad, size_t *first_bad, void *stream);
 
 /* Device-pointer entry points: n_states states already resident in device
    memory in the handle's data order, launched on `stream` (a hipStream_t,
@@ -218,9 +273,20 @@ int pyjac_hip_jacobian_dev(pyjac_handle *h, size_t n_states,
    kernels on C-order chunks of perturbed states. */
 int pyjac_hip_set_fd_options(pyjac_handle *h, int mode, int order, double rtol, double atol);
 
-/* Introspection (Kernel::numSpecies etc., kernel.hpp.in:180-268). */
+/* Introspection (Kernel::numSpecies etc., kernel.hpp.in:101-268). */
 int pyjac_hip_num_species(const pyjac_handle *h);
 int pyjac_hip_num_reactions(const pyjac_handle *h);
+/* Kernel::speciesNames()[i] / reactionStrings()[i] (kernel.hpp.in:101-143):
+   NULL when i is out of range or the tables carried no names (version 1
+   tables).  The strings belong to the handle. */
+const char *pyjac_hip_species_name(const pyjac_handle *h, int i);
+const char *pyjac_hip_reaction_string(const pyjac_handle *h, int i);
+/* Kernel::order() (kernel.hpp.in:122): 'C' or 'F' (0 for a null handle). */
+char pyjac_hip_order(const pyjac_handle *h);
+/* Kernel::vectorWidth() (kernel.hpp.in:139-147): states one kernel
+   instance evaluates together.  The block-per-state kernels give each
+   state its own workgroup, so 1. */
+int pyjac_hip_vector_width(const pyjac_handle *h);
 int pyjac_hip_jacobian_size(const pyjac_handle *h); /* NN*NN or nnz */
 /* Device workspace bytes per state of the Jacobian / rates kernels
    (Kernel::requiredMemorySize, kernel.hpp.in:220). */

@@ -36,6 +36,8 @@ def lib():
         P = ctypes.c_void_p
         L.oracle_species_rates.argtypes = [P, ctypes.c_int, ctypes.c_long, P, P, P, ctypes.c_int]
         L.oracle_species_rates_full.argtypes = [P, ctypes.c_int, ctypes.c_long, P, P, P, P, P, P, P]
+        L.oracle_species_rates_ld.argtypes = [P, ctypes.c_int, ctypes.c_long, P, P, P, P,
+                                              ctypes.c_int]
         L.oracle_chem_utils.argtypes = [P, ctypes.c_int, ctypes.c_long, P, P, P, P]
         L.oracle_jacobian.argtypes = [P, ctypes.c_int, ctypes.c_long, P, P, P, ctypes.c_int]
         L.oracle_jacobian_ad.argtypes = [P, ctypes.c_int, ctypes.c_long, P, P, P]
@@ -67,6 +69,20 @@ def species_rates(mech, conp, param, phi, nthreads=0):
                                     phi.shape[0], _p(param), _p(phi), _p(out), int(nthreads))
     assert st == 0, st
     return out
+
+
+def species_rates_ld(mech, conp, param, phi, nthreads=0):
+    """The species rates evaluated in x87 extended precision from the same
+    binary64 inputs: (hi, lo) with hi + lo the extended result, ``[n, NN]``
+    each."""
+    param, phi = _prep(mech, param, phi)
+    hi = np.empty_like(phi)
+    lo = np.empty_like(phi)
+    st = lib().oracle_species_rates_ld(ctypes.cast(mech.pointer(), ctypes.c_void_p), int(conp),
+                                       phi.shape[0], _p(param), _p(phi), _p(hi), _p(lo),
+                                       int(nthreads))
+    assert st == 0, st
+    return hi, lo
 
 
 def species_rates_full(mech, conp, param, phi):

@@ -125,6 +125,7 @@ template <int N> Dual<N> fmin(double a, const Dual<N>& b) { return fmin(Dual<N>(
 template <int N> Dual<N> fmax(double a, const Dual<N>& b) { return fmax(Dual<N>(a), b); }
 
 inline double value(double x) { return x; }
+inline double value(long double x) { return (double)x; }
 template <int N> double value(const Dual<N>& x) { return x.v; }
 
 using std::exp;
@@ -1239,18 +1240,55 @@ int oracle_species_rates_full(const pyjac_mech_tables* m, int conp, long n,
                               double* pmod, double* rop_net) {
     if (!m) return PYJAC_ERR_ARG;
     const int NN = m->n_species + 1, Nr = m->n_reactions;
-    Work<double> w;
-    w.init(*m);
-    for (long s = 0; s < n; ++s) {
-        species_rates_state<double>(*m, conp, param[s], phi + s * NN, dphi + s * NN, w);
-        for (int r = 0; r < Nr; ++r) {
-            rop_fwd[s * Nr + r] = w.rop_fwd[r];
-            int ri = m->rev_mask[r], ti = m->thd_mask[r];
-            rop_rev[s * Nr + r] = ri >= 0 ? w.rop_rev[ri] : 0.0;
-            pmod[s * Nr + r] = ti >= 0 ? w.pres_mod[ti] : 1.0;
-            rop_net[s * Nr + r] = w.rop_net[r];
+    struct Fn {
+        const pyjac_mech_tables* m; int conp, NN, Nr;
+        const double *param, *phi; double *dphi, *rop_fwd, *rop_rev, *pmod, *rop_net;
+        Work<double> make_state() const { Work<double> w; w.init(*m); return w; }
+        void operator()(long s, Work<double>& w) const {
+            species_rates_state<double>(*m, conp, param[s], phi + s * NN, dphi + s * NN, w);
+            for (int r = 0; r < Nr; ++r) {
+                rop_fwd[s * Nr + r] = w.rop_fwd[r];
+                int ri = m->rev_mask[r], ti = m->thd_mask[r];
+                rop_rev[s * Nr + r] = ri >= 0 ? w.rop_rev[ri] : 0.0;
+                pmod[s * Nr + r] = ti >= 0 ? w.pres_mod[ti] : 1.0;
+                rop_net[s * Nr + r] = w.rop_net[r];
+            }
         }
-    }
+    } fn{m, conp, NN, Nr, param, phi, dphi, rop_fwd, rop_rev, pmod, rop_net};
+    parallel_states(n, 0, fn);
+    return PYJAC_OK;
+}
+
+// The same species-rate expressions evaluated in x87 extended precision
+// (long double, 64-bit significand: 11 bits beyond binary64), inputs and
+// tables as given in binary64.  The result is returned as an unevaluated
+// sum hi + lo of two doubles.  A reference of higher precision for the
+// parity tests: how far a binary64 evaluation (this checker's, or the
+// GPU's) lies from the exact value of the same formulas.
+int oracle_species_rates_ld(const pyjac_mech_tables* m, int conp, long n, const double* param,
+                            const double* phi, double* dphi_hi, double* dphi_lo,
+                            int nthreads) {
+    if (!m || !param || !phi || !dphi_hi || !dphi_lo) return PYJAC_ERR_ARG;
+    const int NN = m->n_species + 1;
+    struct St { Work<long double> w; std::vector<long double> x, y; };
+    struct Fn {
+        const pyjac_mech_tables* m; int conp, NN;
+        const double *param, *phi; double *hi, *lo;
+        St make_state() const {
+            St st; st.w.init(*m); st.x.assign(NN, 0.0L); st.y.assign(NN, 0.0L); return st;
+        }
+        void operator()(long s, St& st) const {
+            for (int i = 0; i < NN; ++i) st.x[i] = phi[s * NN + i];
+            species_rates_state<long double>(*m, conp, (long double)param[s], st.x.data(),
+                                             st.y.data(), st.w);
+            for (int i = 0; i < NN; ++i) {
+                const double h = (double)st.y[i];
+                hi[s * NN + i] = h;
+                lo[s * NN + i] = (double)(st.y[i] - (long double)h);
+            }
+        }
+    } fn{m, conp, NN, param, phi, dphi_hi, dphi_lo};
+    parallel_states(n, nthreads, fn);
     return PYJAC_OK;
 }
 
@@ -1261,6 +1299,7 @@ int oracle_chem_utils(const pyjac_mech_tables* m, int conp, long n,
                       double* bout) {
     if (!m) return PYJAC_ERR_ARG;
     const int Ns = m->n_species, NN = Ns + 1;
+#pragma omp parallel for schedule(static)
     for (long s = 0; s < n; ++s) {
         double T = clampT(phi[s * NN]);
         double logT = log(T), Tinv = 1.0 / T;

@@ -136,6 +136,13 @@ hipError_t launch_jacobian_global(const DevMech& m, const BlockArgs* args, const
                                   const double* phi, double* jac, double* chunk_buf,
                                   long chunk_states, size_t n_contrib, hipStream_t stream);
 
+// ---- finiteness scan (pyjac_checks.hip): flags[s] = 1 for every state s of
+// a (width values per state, data order `order`) holding a NaN or an
+// infinity (flags may be null); counters[0] += their number, counters[1] =
+// min(counters[1], smallest such state)
+hipError_t launch_nonfinite(long n, int width, const double* a, char order, int32_t* flags,
+                            unsigned long long* counters, hipStream_t stream);
+
 // ---- ragged batch: several handles' states in one block-kernel launch
 constexpr int RAGGED_MAX = 8;
 struct RaggedLaunch {

@@ -1,7 +1,7 @@
 // pyjac_block.hip -- block-per-state species-rate and Jacobian kernels (gfx950).
 //
 // One workgroup of WPS wavefronts (WPS = 1, 2, 4 or 8) evaluates one
-// thermochemical state at a time and walks the batch persistently.  All
+// thermochemical state (one launch, a workgroup per state).  All
 // arrays a state needs at random access -- concentrations, Gibbs terms,
 // production rates, third-body / falloff factors and, for the Jacobian, a
 // dense (Ns+1)^2 accumulator -- live in the workgroup's LDS, so HBM sees only

@@ -18,6 +18,7 @@
 #include <vector>
 #include <type_traits>
 #include <cstddef>
+#include <cmath>
 
 #include "../../include/pyjac_hip.h"
 #include "device_mech.h"
@@ -107,6 +108,12 @@ struct pyjac_handle {
     int grid_rates = 0, grid_jac = 0, block = 256;
     int variant[4] = {0, 0, 0, 0};
     hipStream_t stream = nullptr;
+    // names of the tables (version 2), for the introspection entry points
+    std::vector<std::string> species_names, reaction_strings;
+    // device-side finiteness scans: the input check of the device-pointer
+    // entry points and the per-state report; two counters on the device
+    int input_check = 0;
+    unsigned long long* d_counters = nullptr;
 };
 
 namespace {
@@ -510,6 +517,54 @@ int ensure_staging(pyjac_handle* h) {
     return PYJAC_OK;
 }
 
+// host arrays param[N] and phi[N * NN]: the first state holding a NaN or an
+// infinity is an argument error
+int check_host_inputs(const pyjac_handle* h, size_t N, const double* param, const double* phi) {
+    const size_t NN = h->m.NN;
+    for (size_t s = 0; s < N; ++s)
+        if (!std::isfinite(param[s]))
+            return fail(PYJAC_ERR_ARG, "param of state %zu is not finite", s);
+    for (size_t i = 0; i < N * NN; ++i)
+        if (!std::isfinite(phi[i])) {
+            const size_t s = h->order == 'C' ? i / NN : i % N;
+            return fail(PYJAC_ERR_ARG, "phi of state %zu is not finite (NaN or infinity)", s);
+        }
+    return PYJAC_OK;
+}
+
+// scan of a device array (n states, width values each) on stream s: number
+// of non-finite states and the first one (n when none); synchronises s
+int scan_nonfinite(pyjac_handle* h, size_t n, const double* d, int width, int32_t* flags,
+                   size_t* n_bad, size_t* first, hipStream_t s) {
+    if (!h->d_counters) HIP_TRY(hipMalloc(&h->d_counters, 2 * sizeof(unsigned long long)));
+    const unsigned long long init[2] = {0ull, ~0ull};
+    HIP_TRY(hipMemcpyAsync(h->d_counters, init, sizeof(init), hipMemcpyHostToDevice, s));
+    HIP_TRY(pjk_host::launch_nonfinite((long)n, width, d, width == 1 ? 'C' : h->order, flags,
+                                       h->d_counters, s));
+    unsigned long long got[2];
+    HIP_TRY(hipMemcpyAsync(got, h->d_counters, sizeof(got), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    if (n_bad) *n_bad = (size_t)got[0];
+    if (first) *first = got[0] ? (size_t)got[1] : n;
+    return PYJAC_OK;
+}
+
+// the input check of the device-pointer entry points
+int check_dev_inputs(pyjac_handle* h, size_t n, const double* param, const double* phi,
+                     hipStream_t s) {
+    if (!h->input_check || !n) return PYJAC_OK;
+    size_t bad = 0, first = 0;
+    int rc = scan_nonfinite(h, n, param, 1, nullptr, &bad, &first, s);
+    if (rc) return rc;
+    if (bad) return fail(PYJAC_ERR_ARG, "param of state %zu is not finite (%zu states)", first, bad);
+    rc = scan_nonfinite(h, n, phi, h->m.NN, nullptr, &bad, &first, s);
+    if (rc) return rc;
+    if (bad)
+        return fail(PYJAC_ERR_ARG, "phi of state %zu is not finite (NaN or infinity; %zu states)",
+                    first, bad);
+    return PYJAC_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -541,6 +596,8 @@ constexpr int PJ_TABLE_SCALARS = 18;
 struct LoadedTables {
     pyjac_mech_tables t;
     std::vector<std::vector<char>> blocks;
+    std::vector<std::string> names[2];      // species, reactions
+    std::vector<const char*> name_ptrs[2];
 };
 
 int pyjac_hip_tables_load(const char* path, pyjac_mech_tables** out) {
@@ -564,9 +621,10 @@ int pyjac_hip_tables_load(const char* path, pyjac_mech_tables** out) {
     if (!rc) {
         int32_t* dst = &L->t.abi_version;  // the 18 leading int32 fields are contiguous
         for (int i = 0; i < PJ_TABLE_SCALARS; ++i) dst[i] = sc[i];
-        if (L->t.abi_version != PYJAC_MECH_ABI_VERSION)
-            rc = fail(PYJAC_ERR_ARG, "tables file %s: ABI %d, library expects %d", path,
+        if (L->t.abi_version != PYJAC_MECH_ABI_VERSION && L->t.abi_version != 1)
+            rc = fail(PYJAC_ERR_ARG, "tables file %s: ABI %d, library reads 1 and %d", path,
                       L->t.abi_version, PYJAC_MECH_ABI_VERSION);
+        L->t.abi_version = PYJAC_MECH_ABI_VERSION;  // the loaded struct has the name fields
     }
     int idx = 0;
 #define PJ_LOAD(T, name)                                                                  \
@@ -589,6 +647,39 @@ int pyjac_hip_tables_load(const char* path, pyjac_mech_tables** out) {
 #undef PJ_LOAD
     if (!rc && idx != na)
         rc = fail(PYJAC_ERR_ARG, "tables file %s: %d arrays, this library reads %d", path, na, idx);
+    // optional trailer: "PJNAMES1", then the species names and the reaction
+    // strings, each list as an int32 count and (int32 length, bytes) per name
+    char tmagic[8];
+    if (!rc && rd(tmagic, 8)) {
+        if (std::memcmp(tmagic, "PJNAMES1", 8) != 0)
+            rc = fail(PYJAC_ERR_ARG, "tables file %s: unknown trailer", path);
+        const int32_t want[2] = {L->t.n_species, L->t.n_reactions};
+        for (int q = 0; q < 2 && !rc; ++q) {
+            int32_t cnt = 0;
+            if (!rd(&cnt, 4) || cnt != want[q]) {
+                rc = fail(PYJAC_ERR_IO, "tables file %s: name list %d has %d entries, expected %d",
+                          path, q, cnt, want[q]);
+                break;
+            }
+            for (int32_t i = 0; i < cnt && !rc; ++i) {
+                int32_t len = 0;
+                if (!rd(&len, 4) || len < 0 || len > (1 << 20)) {
+                    rc = fail(PYJAC_ERR_IO, "tables file %s: name %d truncated", path, i);
+                    break;
+                }
+                std::string nm((size_t)len, '\0');
+                if (len && !rd(&nm[0], (size_t)len))
+                    rc = fail(PYJAC_ERR_IO, "tables file %s: name %d truncated", path, i);
+                L->names[q].push_back(nm);
+            }
+        }
+        if (!rc) {
+            for (int q = 0; q < 2; ++q)
+                for (const auto& nm : L->names[q]) L->name_ptrs[q].push_back(nm.c_str());
+            L->t.species_names = L->name_ptrs[0].data();
+            L->t.reaction_strings = L->name_ptrs[1].data();
+        }
+    }
     std::fclose(fp);
     if (rc) {
         delete L;
@@ -649,9 +740,9 @@ int pyjac_hip_create(const pyjac_mech_tables* mt, int conp, char order, int jac_
                      int jac_type, pyjac_handle** out) {
     if (!mt || !out) return fail(PYJAC_ERR_ARG, "null argument");
     *out = nullptr;
-    if (mt->abi_version != PYJAC_MECH_ABI_VERSION)
-        return fail(PYJAC_ERR_ARG, "mechanism table ABI %d, library expects %d", mt->abi_version,
-                    PYJAC_MECH_ABI_VERSION);
+    if (mt->abi_version != PYJAC_MECH_ABI_VERSION && mt->abi_version != 1)
+        return fail(PYJAC_ERR_ARG, "mechanism table ABI %d, library reads 1 and %d",
+                    mt->abi_version, PYJAC_MECH_ABI_VERSION);
     if (order != 'C' && order != 'F') return fail(PYJAC_ERR_ARG, "order must be 'C' or 'F'");
     if (jac_format != PYJAC_JAC_FULL && jac_format != PYJAC_JAC_SPARSE)
         return fail(PYJAC_ERR_ARG, "bad jac_format %d", jac_format);
@@ -664,6 +755,16 @@ int pyjac_hip_create(const pyjac_mech_tables* mt, int conp, char order, int jac_
         return fail(PYJAC_ERR_UNSUPPORTED, "Chebyshev degree above 16");
 
     pyjac_handle* h = new pyjac_handle();
+    if (mt->abi_version >= 2) {
+        // names are optional; a table without them gives NULL names
+        if (mt->species_names)
+            for (int k = 0; k < Ns; ++k)
+                h->species_names.emplace_back(mt->species_names[k] ? mt->species_names[k] : "");
+        if (mt->reaction_strings)
+            for (int r = 0; r < Nr; ++r)
+                h->reaction_strings.emplace_back(mt->reaction_strings[r] ? mt->reaction_strings[r]
+                                                                         : "");
+    }
     h->conp = conp ? 1 : 0;
     h->order = order;
     h->jac_format = jac_format;
@@ -1112,6 +1213,7 @@ int pyjac_hip_destroy(pyjac_handle* h) {
     (void)hipFree(h->d_out);
     (void)hipFree(h->d_out2);
     (void)hipFree(h->d_out3);
+    (void)hipFree(h->d_counters);
     delete h;
     return PYJAC_OK;
 }
@@ -1126,13 +1228,32 @@ int pyjac_hip_resize(pyjac_handle* h, size_t problem_size, size_t work_size) {
 int pyjac_hip_species_rates_dev(pyjac_handle* h, size_t n, const double* param, const double* phi,
                                 double* dphi, void* stream) {
     if (!h || (n && (!param || !phi || !dphi))) return fail(PYJAC_ERR_ARG, "null argument");
+    int rc = check_dev_inputs(h, n, param, phi, (hipStream_t)stream);
+    if (rc) return rc;
     return run_rates(h, (long)n, param, phi, dphi, (hipStream_t)stream);
 }
 
 int pyjac_hip_jacobian_dev(pyjac_handle* h, size_t n, const double* param, const double* phi,
                            double* jac, void* stream) {
     if (!h || (n && (!param || !phi || !jac))) return fail(PYJAC_ERR_ARG, "null argument");
+    int rc = check_dev_inputs(h, n, param, phi, (hipStream_t)stream);
+    if (rc) return rc;
     return run_jac(h, (long)n, param, phi, jac, (hipStream_t)stream);
+}
+
+int pyjac_hip_set_input_check(pyjac_handle* h, int on) {
+    if (!h) return fail(PYJAC_ERR_ARG, "null handle");
+    h->input_check = on ? 1 : 0;
+    return PYJAC_OK;
+}
+
+int pyjac_hip_nonfinite_dev(pyjac_handle* h, size_t n, const double* d_arr, int width,
+                            int32_t* d_flags, size_t* n_bad, size_t* first_bad, void* stream) {
+    if (!h || (n && !d_arr) || width < 1) return fail(PYJAC_ERR_ARG, "bad argument");
+    if (n_bad) *n_bad = 0;
+    if (first_bad) *first_bad = n;
+    if (!n) return PYJAC_OK;
+    return scan_nonfinite(h, n, d_arr, width, d_flags, n_bad, first_bad, (hipStream_t)stream);
 }
 
 int pyjac_hip_species_rates(pyjac_handle* h, const double* param, const double* phi,
@@ -1141,6 +1262,7 @@ int pyjac_hip_species_rates(pyjac_handle* h, const double* param, const double* 
     int rc = ensure_staging(h);
     if (rc) return rc;
     const size_t N = h->problem_size, NN = h->m.NN, per = h->staged_states;
+    if ((rc = check_host_inputs(h, N, param, phi))) return rc;
     for (size_t off = 0; off < N; off += per) {
         size_t cnt = std::min(per, N - off);
         if ((rc = copy_states(h, h->d_param, param, N, off, cnt, 1, true, 0))) return rc;
@@ -1158,6 +1280,7 @@ int pyjac_hip_jacobian(pyjac_handle* h, const double* param, const double* phi, 
     if (rc) return rc;
     const size_t N = h->problem_size, NN = h->m.NN, per = h->staged_states;
     const size_t nv = nvalues(h);
+    if ((rc = check_host_inputs(h, N, param, phi))) return rc;
     for (size_t off = 0; off < N; off += per) {
         size_t cnt = std::min(per, N - off);
         if ((rc = copy_states(h, h->d_param, param, N, off, cnt, 1, true, 0))) return rc;
@@ -1169,12 +1292,16 @@ int pyjac_hip_jacobian(pyjac_handle* h, const double* param, const double* phi, 
     return PYJAC_OK;
 }
 
-int pyjac_hip_chem_utils(pyjac_handle* h, const double* phi, double* energy, double* heat,
-                         double* b) {
+int pyjac_hip_chem_utils(pyjac_handle* h, const double* phi, double* b, double* energy,
+                         double* heat) {
     if (!h || !phi || !energy || !heat || !b) return fail(PYJAC_ERR_ARG, "null argument");
     int rc = ensure_staging(h);
     if (rc) return rc;
     const size_t N = h->problem_size, NN = h->m.NN, Ns = h->m.Ns, per = h->staged_states;
+    for (size_t i = 0; i < N * NN; ++i)
+        if (!std::isfinite(phi[i]))
+            return fail(PYJAC_ERR_ARG, "phi of state %zu is not finite",
+                        h->order == 'C' ? i / NN : i % N);
     for (size_t off = 0; off < N; off += per) {
         size_t cnt = std::min(per, N - off);
         if ((rc = copy_states(h, h->d_phi, phi, N, off, cnt, NN, true, 0))) return rc;
@@ -1188,9 +1315,65 @@ int pyjac_hip_chem_utils(pyjac_handle* h, const double* phi, double* energy, dou
     return PYJAC_OK;
 }
 
+// the size checks of the _sized entry points, then the plain call
+static int sized_prepare(pyjac_handle* h, size_t n, size_t got_param, size_t got_phi,
+                         size_t got_out, size_t out_width, const char* out_name) {
+    if (!h) return fail(PYJAC_ERR_ARG, "null handle");
+    const size_t NN = h->m.NN;
+    if (got_param < n)
+        return fail(PYJAC_ERR_ARG, "param holds %zu values, %zu states need %zu", got_param, n, n);
+    if (got_phi < n * NN)
+        return fail(PYJAC_ERR_ARG, "phi holds %zu values, %zu states need %zu", got_phi, n,
+                    n * NN);
+    if (got_out < n * out_width)
+        return fail(PYJAC_ERR_ARG, "%s holds %zu values, %zu states need %zu", out_name, got_out,
+                    n, n * out_width);
+    if (n != h->problem_size) return pyjac_hip_resize(h, n, h->chunk);
+    return PYJAC_OK;
+}
+
+int pyjac_hip_species_rates_sized(pyjac_handle* h, size_t n, const double* param,
+                                  size_t param_count, const double* phi, size_t phi_count,
+                                  double* dphi, size_t dphi_count) {
+    int rc = sized_prepare(h, n, param_count, phi_count, dphi_count, h ? h->m.NN : 0, "dphi");
+    if (rc) return rc;
+    if (!n) return PYJAC_OK;
+    return pyjac_hip_species_rates(h, param, phi, dphi);
+}
+
+int pyjac_hip_jacobian_sized(pyjac_handle* h, size_t n, const double* param, size_t param_count,
+                             const double* phi, size_t phi_count, double* jac, size_t jac_count) {
+    int rc = sized_prepare(h, n, param_count, phi_count, jac_count, h ? nvalues(h) : 0, "jac");
+    if (rc) return rc;
+    if (!n) return PYJAC_OK;
+    return pyjac_hip_jacobian(h, param, phi, jac);
+}
+
+int pyjac_hip_chem_utils_sized(pyjac_handle* h, size_t n, const double* phi, size_t phi_count,
+                               double* b, double* energy, double* heat, size_t out_count) {
+    int rc = sized_prepare(h, n, n, phi_count, out_count, h ? h->m.Ns : 0, "each output");
+    if (rc) return rc;
+    if (!n) return PYJAC_OK;
+    return pyjac_hip_chem_utils(h, phi, b, energy, heat);
+}
+
 int pyjac_hip_num_species(const pyjac_handle* h) { return h ? h->m.Ns : -1; }
 int pyjac_hip_num_reactions(const pyjac_handle* h) { return h ? h->m.Nr : -1; }
 int pyjac_hip_jacobian_size(const pyjac_handle* h) { return h ? nvalues(h) : -1; }
+
+const char* pyjac_hip_species_name(const pyjac_handle* h, int i) {
+    if (!h || i < 0 || i >= (int)h->species_names.size()) return nullptr;
+    return h->species_names[i].c_str();
+}
+
+const char* pyjac_hip_reaction_string(const pyjac_handle* h, int i) {
+    if (!h || i < 0 || i >= (int)h->reaction_strings.size()) return nullptr;
+    return h->reaction_strings[i].c_str();
+}
+
+char pyjac_hip_order(const pyjac_handle* h) { return h ? h->order : 0; }
+
+int pyjac_hip_vector_width(const pyjac_handle* h) { return h ? 1 : -1; }
 
 size_t pyjac_hip_required_memory(const pyjac_handle* h) {
     if (!h) return 0;

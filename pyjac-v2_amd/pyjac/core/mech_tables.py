@@ -13,7 +13,7 @@ from .enum_types import RateSpecialization, JacobianType
 from .mech_interpret import load_mechanism
 from .rate_tables import determine_jac_inds, named_tables
 
-ABI_VERSION = 1
+ABI_VERSION = 2
 
 ARR_A, ARR_AT_B, ARR_FULL, ARR_NO_TA, ARR_NO_B = range(5)
 
@@ -50,7 +50,10 @@ _FIELDS = [
     ('rxn_has_ns', _I),
     ('ccs_col_ptr', _I), ('ccs_row_ind', _I), ('crs_row_ptr', _I),
     ('crs_col_ind', _I),
+    ('species_names', ctypes.POINTER(ctypes.c_char_p)),
+    ('reaction_strings', ctypes.POINTER(ctypes.c_char_p)),
 ]
+_NAME_FIELDS = ('species_names', 'reaction_strings')
 
 
 class CMechTables(ctypes.Structure):
@@ -219,6 +222,12 @@ class MechTables(object):
             if name in A:
                 arr = A[name]
                 setattr(st, name, arr.ctypes.data_as(ctype))
+        # the names the introspection entry points return (kept alive here)
+        self._names_c = []
+        for field, names in zip(_NAME_FIELDS, (self.species_names, self.reaction_strings)):
+            arr = (ctypes.c_char_p * max(1, len(names)))(*[n.encode() for n in names])
+            self._names_c.append(arr)
+            setattr(st, field, ctypes.cast(arr, ctypes.POINTER(ctypes.c_char_p)))
         self.cstruct = st
 
     def pointer(self):
@@ -226,15 +235,19 @@ class MechTables(object):
         return ctypes.pointer(self.cstruct)
 
     TABLES_MAGIC = b'PJHIPTB1'
+    NAMES_MAGIC = b'PJNAMES1'
 
     def save_binary(self, path):
         """Write the tables as a binary file that C and C++ callers load with
         ``pyjac_hip_tables_load`` (include/pyjac_hip.h): the magic
         ``PJHIPTB1``, the number of scalar fields and their int32 values, the
         number of arrays, then per array (in the struct's field order) its
-        element type (int32 0 / float64 1), int64 length and data."""
+        element type (int32 0 / float64 1), int64 length and data; then the
+        trailer ``PJNAMES1`` with the species names and the reaction strings
+        (per list an int32 count, per name an int32 length and its bytes)."""
         scalars = [(n, t) for n, t in _FIELDS if t is ctypes.c_int32]
-        arrays = [(n, t) for n, t in _FIELDS if t is not ctypes.c_int32]
+        arrays = [(n, t) for n, t in _FIELDS
+                  if t is not ctypes.c_int32 and n not in _NAME_FIELDS]
         with open(path, 'wb') as f:
             f.write(self.TABLES_MAGIC)
             f.write(np.int32(len(scalars)).tobytes())
@@ -247,6 +260,14 @@ class MechTables(object):
                 f.write(np.uint8(1 if is_d else 0).tobytes())
                 f.write(np.int64(a.size).tobytes())
                 f.write(a.tobytes())
+            # trailer: species names and reaction strings
+            f.write(self.NAMES_MAGIC)
+            for names in (self.species_names, self.reaction_strings):
+                f.write(np.int32(len(names)).tobytes())
+                for nm in names:
+                    b = nm.encode()
+                    f.write(np.int32(len(b)).tobytes())
+                    f.write(b)
         return path
 
     # sparse index helpers --------------------------------------------------

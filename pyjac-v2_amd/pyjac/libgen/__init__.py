@@ -19,14 +19,13 @@ LIB_NAME = 'libpyjac_hip.so'
 LIB_PATH = os.path.join(PKG_DIR, LIB_NAME)
 
 SOURCES = ['pyjac_kernels.hip', 'pyjac_wave.hip', 'pyjac_block.hip', 'pyjac_fd.hip',
-           'pyjac_integrator.hip', 'surrogate_kernels.hip', 'runtime.cpp',
-           'surrogate_runtime.cpp']
+           'pyjac_checks.hip', 'pyjac_integrator.hip', 'surrogate_kernels.hip',
+           'runtime.cpp']
 HEADERS = ['device_mech.h', 'kernels.h']
 
 # every HIP translation unit is compiled with packed FP32 disabled
 HIPCC_FLAGS = ['--offload-arch=gfx950', '-O3', '-std=c++17', '-fPIC',
-               '-shared', '-Xclang', '-target-feature', '-Xclang',
-               '-packed-fp32-ops']
+               '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops']
 
 
 def hipcc():
@@ -36,19 +35,35 @@ def hipcc():
     return path
 
 
-def build_library(out_path=LIB_PATH, extra_flags=(), verbose=False):
-    """Compile ``libpyjac_hip.so`` into ``out_path`` (always recompiles)."""
-    srcs = [os.path.join(CSRC, s) for s in SOURCES
-            if os.path.exists(os.path.join(CSRC, s))]
-    tmp = out_path + '.tmp'
-    cmd = [hipcc()] + HIPCC_FLAGS + list(extra_flags) + \
-        ['-I', INCLUDE, '-o', tmp] + srcs
-    res = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
-                         text=True)
-    if res.returncode != 0:
-        raise RuntimeError('hipcc failed:\n' + res.stdout)
-    if verbose:
-        print(res.stdout)
+def build_library(out_path=LIB_PATH, extra_flags=(), verbose=False, jobs=8):
+    """Compile ``libpyjac_hip.so`` into ``out_path`` (always recompiles):
+    every source to an object in parallel, then one link."""
+    import concurrent.futures
+    import tempfile
+    for s in SOURCES:
+        if not os.path.exists(os.path.join(CSRC, s)):
+            raise RuntimeError('missing source ' + s)
+    with tempfile.TemporaryDirectory(prefix='pyjac_build_') as tmpdir:
+        def compile_one(src):
+            obj = os.path.join(tmpdir, os.path.basename(src) + '.o')
+            cmd = [hipcc()] + HIPCC_FLAGS + list(extra_flags) + \
+                ['-I', INCLUDE, '-c', os.path.join(CSRC, src), '-o', obj]
+            res = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                                 text=True)
+            return obj, res
+        with concurrent.futures.ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
+            results = list(ex.map(compile_one, SOURCES))
+        for obj, res in results:
+            if res.returncode != 0:
+                raise RuntimeError('hipcc failed:\n' + res.stdout)
+            if verbose:
+                print(res.stdout)
+        tmp = out_path + '.tmp'
+        cmd = [hipcc(), '--offload-arch=gfx950', '-shared', '-fPIC', '-o', tmp] + \
+            [obj for obj, _ in results]
+        res = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+        if res.returncode != 0:
+            raise RuntimeError('link failed:\n' + res.stdout)
     os.replace(tmp, out_path)
     if os.path.basename(out_path) == LIB_NAME:
         build_main(os.path.join(os.path.dirname(out_path), MAIN_NAME), os.path.dirname(out_path))

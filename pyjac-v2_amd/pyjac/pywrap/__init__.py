@@ -50,6 +50,18 @@ def load_library(path=None):
     for name in ('pyjac_hip_species_rates', 'pyjac_hip_jacobian'):
         getattr(L, name).argtypes = [P, P, P, P]
     L.pyjac_hip_chem_utils.argtypes = [P, P, P, P, P]
+    for name in ('pyjac_hip_species_rates_sized', 'pyjac_hip_jacobian_sized'):
+        getattr(L, name).argtypes = [P, S, P, S, P, S, P, S]
+    L.pyjac_hip_chem_utils_sized.argtypes = [P, S, P, S, P, P, P, S]
+    L.pyjac_hip_set_input_check.argtypes = [P, I]
+    L.pyjac_hip_nonfinite_dev.argtypes = [P, S, P, I, P, ctypes.POINTER(S), ctypes.POINTER(S),
+                                          P]
+    for name in ('pyjac_hip_species_name', 'pyjac_hip_reaction_string'):
+        getattr(L, name).argtypes = [P, I]
+        getattr(L, name).restype = ctypes.c_char_p
+    L.pyjac_hip_order.argtypes = [P]
+    L.pyjac_hip_order.restype = ctypes.c_char
+    L.pyjac_hip_vector_width.argtypes = [P]
     for name in ('pyjac_hip_species_rates_dev', 'pyjac_hip_jacobian_dev'):
         getattr(L, name).argtypes = [P, S, P, P, P, P]
     for name in ('pyjac_hip_num_species', 'pyjac_hip_num_reactions',
@@ -143,10 +155,45 @@ class _KernelBase(object):
             self._h = None
 
     def species_names(self):
-        return list(self.mech.species_names)
+        """Species names in the kernel's order (bath gas last), from the
+        library's copy of the tables (``Kernel::speciesNames``)."""
+        return [self.lib.pyjac_hip_species_name(self._h, i).decode()
+                for i in range(self.num_species())]
 
     def reaction_strings(self):
-        return list(self.mech.reaction_strings)
+        return [self.lib.pyjac_hip_reaction_string(self._h, i).decode()
+                for i in range(self.num_reactions())]
+
+    def vector_width(self):
+        return self.lib.pyjac_hip_vector_width(self._h)
+
+    def set_input_check(self, on=True):
+        """Check device inputs for NaN / infinity before each device call
+        (host arrays are always checked)."""
+        check(self.lib.pyjac_hip_set_input_check(self._h, int(bool(on))))
+
+    def nonfinite_states(self, arr, width=None):
+        """Per-state finiteness report of a device array in the kernel's
+        data order: (flags (int32 device tensor, 1 = the state holds a NaN
+        or an infinity), count, first such state or n)."""
+        import torch
+        width = int(width or self._out_width())
+        n = _numel(arr) // width
+        if n * width != _numel(arr):
+            raise ValueError('array size is not a multiple of {}'.format(width))
+        pa, on_dev = _ptr(arr)
+        if not on_dev:
+            raise ValueError('nonfinite_states takes a device array')
+        flags = torch.empty(n, dtype=torch.int32, device=arr.device)
+        bad, first = ctypes.c_size_t(), ctypes.c_size_t()
+        check(self.lib.pyjac_hip_nonfinite_dev(self._h, n, pa, width,
+                                               ctypes.c_void_p(flags.data_ptr()),
+                                               ctypes.byref(bad), ctypes.byref(first),
+                                               _current_stream_ptr()))
+        return flags, int(bad.value), int(first.value)
+
+    def _out_width(self):
+        return self.mech.NN
 
     def num_species(self):
         return self.lib.pyjac_hip_num_species(self._h)
@@ -217,21 +264,32 @@ class _KernelBase(object):
             # a cached choice this build cannot run keeps the automatic one
             self.lib.pyjac_hip_set_variant(self._h, self._variant_slot, int(choice['variant']))
 
-    def _run(self, host_fn, dev_fn, param, phi, out):
+    def _check_sizes(self, param, phi, out):
+        """n from param; phi must hold n * NN values and out n * (width of
+        the kernel's output), or ValueError before anything is called."""
+        n = _numel(param)
+        NN, w = self.mech.NN, self._out_width()
+        for a, need, name in ((phi, n * NN, 'phi'), (out, n * w, 'output')):
+            if _numel(a) != need:
+                raise ValueError('{} holds {} values; {} states need {} ({} per state)'
+                                 .format(name, _numel(a), n, need, need // max(n, 1)))
+        if isinstance(out, np.ndarray) and not out.flags['WRITEABLE']:
+            raise ValueError('output array is not writeable')
+        return n
+
+    def _run(self, sized_fn, dev_fn, param, phi, out):
         pp, dp = _ptr(param)
         ph, dph = _ptr(phi)
         po, do = _ptr(out)
+        n = self._check_sizes(param, phi, out)
         if dp or dph or do:
             if not (dp and dph and do):
                 raise ValueError('mix of host and device arrays')
-            n = _numel(param)
             stream = _current_stream_ptr()
             check(dev_fn(self._h, n, pp, ph, po, stream))
             return out
-        n = _numel(param)
-        if n != self.problem_size:
-            self.resize(n)
-        check(host_fn(self._h, pp, ph, po))
+        check(sized_fn(self._h, n, pp, _numel(param), ph, _numel(phi), po, _numel(out)))
+        self.problem_size = n
         return out
 
 
@@ -317,11 +375,14 @@ class PyJacobianKernel(_KernelBase):
     _variant_slot = 1
 
     def __call__(self, param, phi, jac):
-        return self._run(self.lib.pyjac_hip_jacobian, self.lib.pyjac_hip_jacobian_dev,
+        return self._run(self.lib.pyjac_hip_jacobian_sized, self.lib.pyjac_hip_jacobian_dev,
                          param, phi, jac)
 
     def jacobian_size(self):
         return self.lib.pyjac_hip_jacobian_size(self._h)
+
+    def _out_width(self):
+        return self.jacobian_size()
 
 
 class PySpeciesRatesKernel(_KernelBase):
@@ -330,7 +391,7 @@ class PySpeciesRatesKernel(_KernelBase):
     _variant_slot = 0
 
     def __call__(self, param, phi, dphi):
-        return self._run(self.lib.pyjac_hip_species_rates,
+        return self._run(self.lib.pyjac_hip_species_rates_sized,
                          self.lib.pyjac_hip_species_rates_dev, param, phi, dphi)
 
 
@@ -341,11 +402,22 @@ class PyChemUtilsKernel(_KernelBase):
     _variant_slot = 2
 
     def __call__(self, phi, b, specific_energy, specific_heat):
-        n = _numel(phi) // self.mech.NN
-        if n != self.problem_size:
-            self.resize(n)
-        args = [_ptr(a)[0] for a in (phi, specific_energy, specific_heat, b)]
-        check(self.lib.pyjac_hip_chem_utils(self._h, *args))
+        NN, Ns = self.mech.NN, self.mech.Ns
+        n = _numel(phi) // NN
+        if n * NN != _numel(phi):
+            raise ValueError('phi holds {} values, not a multiple of NN = {}'
+                             .format(_numel(phi), NN))
+        for a, name in ((b, 'b'), (specific_energy, 'specific_energy'),
+                        (specific_heat, 'specific_heat')):
+            if _numel(a) != n * Ns:
+                raise ValueError('{} holds {} values; {} states need {}'
+                                 .format(name, _numel(a), n, n * Ns))
+            if _ptr(a)[1]:
+                raise ValueError('chem utils take host arrays')
+        args = [_ptr(a)[0] for a in (phi, b, specific_energy, specific_heat)]
+        check(self.lib.pyjac_hip_chem_utils_sized(self._h, n, args[0], _numel(phi), args[1],
+                                                  args[2], args[3], n * Ns))
+        self.problem_size = n
         return b, specific_energy, specific_heat
 
 

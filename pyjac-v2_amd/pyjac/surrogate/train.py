@@ -148,12 +148,7 @@ def rates_with_fallback(sur, param, phi):
     torch = _torch()
     colmajor = sur.order == 'F'
     n = int(param.numel())
-    if sur.log10_tol <= -1e29:
-        # the calibration accepted no state at this tolerance: the network
-        # cannot help, so the FP64 kernel takes every state
-        dphi = fp64_rates(sur.mech, sur.conp, param, phi, sur.order)
-        return production_rates(sur, param, phi, dphi, colmajor), \
-            torch.zeros(n, dtype=torch.int32, device=param.device)
+    # the network and its gate always run, whatever the calibration gave
     w, accept = sur.predict(param, phi, n)
     idx = torch.nonzero(accept == 0).flatten()
     m = int(idx.numel())

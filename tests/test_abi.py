@@ -39,7 +39,7 @@ def test_mech_struct_matches_header():
             decl = decl.split('{', 1)[-1].strip()
         if not decl:
             continue
-        decl = re.sub(r'^(const\s+)?(int32_t|double)\s*', '', decl)
+        decl = re.sub(r'^(const\s+)?(int32_t|double|char\s*\*\s*const)\s*', '', decl)
         fields += [f.strip().lstrip('*') for f in decl.split(',')]
     assert fields == [f[0] for f in CMechTables._fields_]
 

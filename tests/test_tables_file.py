@@ -1,7 +1,9 @@
 """The tables file (MechTables.save_binary -> pyjac_hip_tables_load, no GPU
 needed): every scalar and every array comes back bit for bit, in the
-struct's field order, for mechanisms with and without PLOG / Chebyshev /
-falloff tables; damaged files are errors."""
+struct's field order, with the species names and reaction strings of the
+trailer, for mechanisms with and without PLOG / Chebyshev / falloff tables;
+damaged files are errors, and a file without the trailer (the first layout)
+still loads, without names."""
 
 import ctypes
 
@@ -30,7 +32,11 @@ def test_round_trip(mechs, tmp_path, name):
     try:
         t = out.contents
         for field, ctype in _FIELDS:
-            if ctype is ctypes.c_int32:
+            if field in ('species_names', 'reaction_strings'):
+                names = mech.species_names if field == 'species_names' else mech.reaction_strings
+                got = [getattr(t, field)[i].decode() for i in range(len(names))]
+                assert got == names, field
+            elif ctype is ctypes.c_int32:
                 assert getattr(t, field) == getattr(mech.cstruct, field), field
             else:
                 ref = mech.arrays[field]
@@ -51,3 +57,19 @@ def test_damaged_files(mechs, tmp_path):
         _load(tmp_path / 'magic.pjt')
     with pytest.raises(HipRuntimeError, match='could not open'):
         _load(tmp_path / 'none.pjt')
+
+
+def test_file_without_names_trailer(mechs, tmp_path):
+    """A tables file of the first layout (no names trailer) loads, and its
+    struct carries NULL name fields."""
+    mech = mechs('h2o2')
+    good = open(mech.save_binary(str(tmp_path / 'good.pjt')), 'rb').read()
+    cut = good.index(mech.NAMES_MAGIC)
+    (tmp_path / 'v1.pjt').write_bytes(good[:cut])
+    L, out = _load(tmp_path / 'v1.pjt')
+    try:
+        assert not out.contents.species_names
+        assert not out.contents.reaction_strings
+        assert out.contents.n_species == mech.Ns
+    finally:
+        L.pyjac_hip_tables_free(out)

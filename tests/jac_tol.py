@@ -31,9 +31,12 @@ def jacobian_errors(got, ref):
     got = np.asarray(got, dtype=np.float64)
     ref = np.asarray(ref, dtype=np.float64)
     diff = np.abs(got - ref)
-    nrm = np.linalg.norm(ref, axis=1)
-    fro = np.linalg.norm(got - ref, axis=1) / np.where(nrm > 0, nrm, 1.0)
     mx = np.abs(ref).max(axis=1, keepdims=True)
+    # norms of the entries scaled by the state's largest one (entries near
+    # the top of the double range would overflow the sums of squares)
+    sc = np.where(mx > 0, mx, 1.0)
+    nrm = np.linalg.norm(ref / sc, axis=1)
+    fro = np.linalg.norm(diff / sc, axis=1) / np.where(nrm > 0, nrm, 1.0)
     elem = diff / (RTOL * np.abs(ref) + ATOL_REL * mx + 1e-300)
     over = int(np.count_nonzero(diff > 1e-10 * np.abs(ref)))
     return float(fro.max()), float(elem.max()), over

@@ -18,6 +18,7 @@ import pytest
 
 import oracle
 from rate_tol import rate_error, rate_scales
+from jac_tol import assert_jacobian_close, per_state
 from pyjac.core.states import sample_states
 from pyjac.pywrap import PyJacobianKernel, PySpeciesRatesKernel
 
@@ -92,8 +93,5 @@ def test_guards_rates_and_jacobian(mechs, name, conp, order):
         jk = PyJacobianKernel(n, 0, mech=mech, conp=conp, order=order, jac_format=fmt)
         jac = np.zeros_like(jref)
         jk(param, phi_in, jac)
-        a = jac.reshape(n, -1) if order == 'C' else jac.reshape(-1, n).T
-        b = jref.reshape(n, -1) if order == 'C' else jref.reshape(-1, n).T
-        assert np.isfinite(a).all()
-        fro = np.linalg.norm(a - b, axis=1) / np.linalg.norm(b, axis=1)
-        assert fro.max() <= 1e-8, (fmt, fro)
+        assert_jacobian_close(per_state(jac, n, order), per_state(jref, n, order),
+                              'guards {} conp={} {} fmt={}'.format(name, conp, order, fmt))

@@ -264,6 +264,22 @@ int pyjac_hip_jacobian_dev(pyjac_handle *h, size_t n_states,
                            const double *d_param, const double *d_phi,
                            double *d_jac, void *stream);
 
+/* The Jacobian device-pointer entry point without any allocation.  No
+   counterpart in the reference: for callers that cannot call hipMalloc
+   (code under HIP graph capture, tight serving loops, harnesses that forbid
+   it).  Same arguments and results as pyjac_hip_jacobian_dev; the device
+   memory the F-order path (C-order chunks and their transposes) and the
+   device-memory gather variant need beyond the inputs and outputs comes
+   from a fixed scratch area of 64 MiB per GPU inside the library, held from
+   the library's load on a GPU until the process ends.  Every call on one
+   GPU shares that area: calls that overlap in time on one GPU race (issue
+   them on one stream).  PYJAC_ERR_UNSUPPORTED when 64 states' buffers
+   exceed the area (e.g. a full-format F-order Jacobian of 500 species),
+   for the lane- and wave-per-state variants (their per-wave work buffers),
+   finite-difference handles and handles with the input check on. */
+int pyjac_hip_jacobian_dev_noalloc(pyjac_handle *h, size_t n_states, const double *d_param,
+                                   const double *d_phi, double *d_jac, void *stream);
+
 /* Options of a PYJAC_JAC_FINITE_DIFFERENCE handle (finite_difference_jacobian,
    core/create_jacobian.py:4523-4526): mode PYJAC_FD_FORWARD / _BACKWARD
    (order 1-6) or PYJAC_FD_CENTRAL (order 2, 4, 6, 8), and the relative and

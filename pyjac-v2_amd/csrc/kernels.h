@@ -143,6 +143,12 @@ hipError_t launch_jacobian_global(const DevMech& m, const BlockArgs* args, const
 hipError_t launch_nonfinite(long n, int width, const double* a, char order, int32_t* flags,
                             unsigned long long* counters, hipStream_t stream);
 
+// ---- the fixed scratch area of the Jacobian entry point that allocates
+// nothing (pyjac_noalloc.hip): 64 MiB per GPU
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)64 << 20;
+// device address of the scratch area on the current device
+hipError_t noalloc_scratch(double** out);
+
 // ---- ragged batch: several handles' states in one block-kernel launch
 constexpr int RAGGED_MAX = 8;
 struct RaggedLaunch {

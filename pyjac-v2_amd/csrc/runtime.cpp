@@ -1241,6 +1241,59 @@ int pyjac_hip_jacobian_dev(pyjac_handle* h, size_t n, const double* param, const
     return run_jac(h, (long)n, param, phi, jac, (hipStream_t)stream);
 }
 
+// The Jacobian without any allocation: the chunk buffers come from the
+// library's fixed scratch area (pjk_host::noalloc_scratch); the F-order
+// transposes run on the caller's stream after each chunk, one chunk buffer
+int pyjac_hip_jacobian_dev_noalloc(pyjac_handle* h, size_t n, const double* param,
+                                   const double* phi, double* jac, void* stream) {
+    if (!h || (n && (!param || !phi || !jac))) return fail(PYJAC_ERR_ARG, "null argument");
+    if (!n) return PYJAC_OK;
+    if (h->input_check)
+        return fail(PYJAC_ERR_UNSUPPORTED, "the input check synchronises and is not available "
+                                           "without allocation; turn it off");
+    if (h->jac_type == PYJAC_JAC_FINITE_DIFFERENCE)
+        return fail(PYJAC_ERR_UNSUPPORTED, "finite-difference Jacobians need work buffers");
+    hipStream_t s = (hipStream_t)stream;
+    double* buf = nullptr;
+    const size_t cap = pjk_host::NOALLOC_SCRATCH_BYTES / sizeof(double);
+    if (use_block(h, 1)) {
+        if (h->order == 'C') {
+            HIP_TRY(pjk_host::launch_jacobian_block(h->m, h->d_args[1], h->lds[1],
+                                                    block_wps(h, 1), h->conp, 'C', (long)n,
+                                                    param, phi, jac, nullptr, 0, s));
+            return PYJAC_OK;
+        }
+        // F order: C-order chunks of whole multiples of 64 states
+        const size_t per = (size_t)h->m.jac_nnz;
+        const long cs = (long)std::min<size_t>(cap / per, ((size_t)n + 63) / 64 * 64) / 64 * 64;
+        if (cs < 64)
+            return fail(PYJAC_ERR_UNSUPPORTED,
+                        "64 states need %zu bytes of scratch, the fixed area holds %zu",
+                        64 * per * sizeof(double), pjk_host::NOALLOC_SCRATCH_BYTES);
+        HIP_TRY(pjk_host::noalloc_scratch(&buf));
+        HIP_TRY(pjk_host::launch_jacobian_block(h->m, h->d_args[1], h->lds[1], block_wps(h, 1),
+                                                h->conp, 'F', (long)n, param, phi, jac, buf, cs,
+                                                s, nullptr));
+        return PYJAC_OK;
+    }
+    if (use_glob(h)) {
+        const size_t per = (size_t)h->bt.n_contrib + (h->order == 'F' ? h->m.jac_nnz : 0);
+        const long cs = (long)std::min<size_t>(cap / per, ((size_t)n + 63) / 64 * 64) / 64 * 64;
+        if (cs < 64)
+            return fail(PYJAC_ERR_UNSUPPORTED,
+                        "64 states need %zu bytes of scratch, the fixed area holds %zu",
+                        64 * per * sizeof(double), pjk_host::NOALLOC_SCRATCH_BYTES);
+        HIP_TRY(pjk_host::noalloc_scratch(&buf));
+        HIP_TRY(pjk_host::launch_jacobian_global(h->m, h->d_args_g, h->lds_g, glob_wps(h),
+                                                 h->conp, h->order, (long)n, param, phi, jac,
+                                                 buf, cs, (size_t)h->bt.n_contrib, s));
+        return PYJAC_OK;
+    }
+    return fail(PYJAC_ERR_UNSUPPORTED,
+                "the handle's Jacobian variant keeps a work buffer per wave (lane- and "
+                "wave-per-state kernels); the block-per-state variants run without allocation");
+}
+
 int pyjac_hip_set_input_check(pyjac_handle* h, int on) {
     if (!h) return fail(PYJAC_ERR_ARG, "null handle");
     h->input_check = on ? 1 : 0;

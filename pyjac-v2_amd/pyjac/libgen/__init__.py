@@ -19,8 +19,8 @@ LIB_NAME = 'libpyjac_hip.so'
 LIB_PATH = os.path.join(PKG_DIR, LIB_NAME)
 
 SOURCES = ['pyjac_kernels.hip', 'pyjac_wave.hip', 'pyjac_block.hip', 'pyjac_fd.hip',
-           'pyjac_checks.hip', 'pyjac_integrator.hip', 'surrogate_kernels.hip',
-           'runtime.cpp']
+           'pyjac_checks.hip', 'pyjac_noalloc.hip', 'pyjac_integrator.hip',
+           'surrogate_kernels.hip', 'runtime.cpp']
 HEADERS = ['device_mech.h', 'kernels.h']
 
 # every HIP translation unit is compiled with packed FP32 disabled

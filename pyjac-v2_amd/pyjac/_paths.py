@@ -2,8 +2,10 @@
 import os
 
 PKG = os.path.dirname(os.path.abspath(__file__))
-REPO = os.path.dirname(os.path.dirname(PKG))
-MECH_DIR = os.path.join(REPO, 'tests', 'golden', 'mechanisms')
+# package data: the reference's bundled mechanisms (data/h2o2.inp,
+# data/grimech30.inp + thermo30.dat, pyjac/tests/test.inp and the .cti files)
+# and their Cantera YAML renderings
+MECH_DIR = os.path.join(PKG, 'data', 'mechanisms')
 
 MECHANISMS = {
     'h2o2': ('h2o2.inp', None),

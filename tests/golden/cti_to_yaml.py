@@ -1,5 +1,5 @@
 """Write Cantera-YAML copies of the CTI test mechanisms
-(tests/golden/mechanisms/{test,h2o2}.cti -> .yaml), with the same numbers in
+(pyjac-v2_amd/pyjac/data/mechanisms/{test,h2o2}.cti -> .yaml), with the same numbers in
 the same units, so that the YAML reader can be checked against the Chemkin
 and CTI inputs of the same mechanisms.  The CTI files are the reference's
 (pyjac/tests/test.cti, data/h2o2.cti); the YAML layout follows Cantera's
@@ -110,7 +110,7 @@ def convert(text):
 
 
 def main():
-    mech = os.path.join(HERE, 'mechanisms')
+    mech = os.path.join(os.path.dirname(HERE), '..', 'pyjac-v2_amd', 'pyjac', 'data', 'mechanisms')
     for name in ('test', 'h2o2'):
         with open(os.path.join(mech, name + '.cti')) as f:
             text = convert(f.read())

@@ -21,7 +21,7 @@ from pyjac.core.mech_tables import MechTables
 from test_cti import _CGS, _PHASE, _compare_arrays, _species_block
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-MECH = os.path.join(ROOT, 'tests', 'golden', 'mechanisms')
+MECH = os.path.join(ROOT, 'pyjac-v2_amd', 'pyjac', 'data', 'mechanisms')
 
 
 @pytest.mark.parametrize('yaml_name,twin', [('test.yaml', 'test.inp'), ('test.yaml', 'test.cti'),

@@ -24,7 +24,7 @@ from pyjac.core.exceptions import MechanismError
 from pyjac.core.mech_tables import MechTables
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-MECH = os.path.join(ROOT, 'tests', 'golden', 'mechanisms')
+MECH = os.path.join(ROOT, 'pyjac-v2_amd', 'pyjac', 'data', 'mechanisms')
 
 
 def _compare_arrays(a, b, rtol=0.0):

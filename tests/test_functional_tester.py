@@ -21,7 +21,7 @@ from pyjac.functional_tester.norms import INF_CUTOFF, JacobianErrors, RateErrors
     jacobian_threshold
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-MECH = os.path.join(ROOT, 'tests', 'golden', 'mechanisms')
+MECH = os.path.join(ROOT, 'pyjac-v2_amd', 'pyjac', 'data', 'mechanisms')
 PASR = os.path.join(ROOT, 'tests', 'golden', 'h2_pasr_states.npz')
 AIR = {'O2': 1.0, 'N2': 3.76}
 

@@ -16,7 +16,7 @@ from pyjac.functional_tester.validate import validate_jacobian, validate_species
 pytestmark = pytest.mark.gpu
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-MECH = os.path.join(ROOT, 'tests', 'golden', 'mechanisms')
+MECH = os.path.join(ROOT, 'pyjac-v2_amd', 'pyjac', 'data', 'mechanisms')
 AIR = {'O2': 1.0, 'N2': 3.76}
 
 

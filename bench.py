@@ -94,21 +94,29 @@ SURROGATE_MODEL = os.path.join(ROOT, 'pyjac-v2_amd', 'pyjac', 'surrogate',
 
 
 def surrogate_leg(mech, n):
-    """The species-rate surrogate with its error gate and FP64 fallback at the
-    1e-3 tolerance, against the FP64 species-rate kernel on n fresh states of
-    the same workload (F order): speed-up, accepted fraction, largest error."""
+    """The species-rate surrogate with its error gate and FP64 fallback
+    against the FP64 species-rate kernel on n fresh states of the same
+    workload (F order), the network always in the timed path: at the 1e-3
+    tolerance, and at each tolerance of a sweep (gate calibrated on other
+    fresh states), reporting the first tolerance at which the gate accepts
+    more than half of the states."""
     if not os.path.exists(SURROGATE_MODEL):
         return None
     from pyjac.surrogate import Surrogate
-    from pyjac.surrogate.train import evaluate
+    from pyjac.surrogate.train import error_percentiles, sweep
     sur = Surrogate(mech, conp=False, order='F').load(SURROGATE_MODEL)
-    r = evaluate(sur, n, seed=4242, tol=1e-3)
-    return {'tol': 1e-3, 'states': n, 'speedup': r['speedup'],
-            'accepted_fraction': r['accepted_fraction'],
-            'max_error_all': r['max_error_all'],
-            'fp64_ms': r['fp64_ms'], 'surrogate_with_fallback_ms': r['surrogate_with_fallback_ms'],
-            'network_only_ms': r['network_only_ms'],
-            'error': 'max_k |w_k - w_ref_k| / max_k |w_ref_k| per state'}
+    rows = sweep(sur, n, n, reps=3)
+    keep = ('tol', 'accepted_fraction', 'speedup', 'max_error_all', 'fp64_ms',
+            'surrogate_with_fallback_ms', 'network_only_ms')
+    first = rows[0]
+    half = next((r for r in rows if r['accepted_fraction'] > 0.5), None)
+    out = {k: first[k] for k in keep}
+    out.update({'states': n,
+                'error': 'max_k |w_k - w_ref_k| / max_k |w_ref_k| per state',
+                'network_error': error_percentiles(sur, n),
+                'by_tolerance': [{k: r[k] for k in keep} for r in rows],
+                'acceptance_above_half': ({k: half[k] for k in keep} if half else None)})
+    return out
 
 
 def main():

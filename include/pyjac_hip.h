@@ -340,13 +340,28 @@ int pyjac_hip_batch_destroy(pyjac_batch *b);
 /*
  * Autotuner (no counterpart in the reference): times every kernel family that
  * can run the handle's mechanism -- block-per-state with 1, 2, 4 or 8 waves
- * per state, wave-per-state, lane-per-state -- on n resident states (results
- * land in d_out) and makes the fastest the handle's automatic choice.
+ * per state and each built occupancy target, wave-per-state, lane-per-state
+ * -- on n resident states (results land in d_out), then, for an F-order
+ * Jacobian of the LDS block kernel, chunks of 16384, 32768 and 131072 states
+ * against the default 65536, and makes the fastest the handle's automatic
+ * choice.
  * Reports the winning variant id and its time per call.
  */
 int pyjac_hip_autotune(pyjac_handle *h, int kernel, size_t n, const double *d_param,
                        const double *d_phi, double *d_out, void *stream, int *best_variant,
                        float *best_ms);
+
+/* The other knobs the autotuner searches (no counterpart in the reference):
+   the occupancy target of the block kernels (waves per SIMD the register
+   allocator is held to; 0 = the default build, 3 for the Jacobian and 5 for
+   the species rates; 2 and 4 are built for the Jacobian with one or two
+   waves per state, 4 and 6 for the one-wave species rates) and the number of
+   states per F-order chunk of the Jacobian (0 = the default; multiples of
+   64).  pyjac_hip_autotune sets both to its choice; get_tuning also reports
+   the waves per state in use. */
+int pyjac_hip_set_tuning(pyjac_handle *h, int kernel, int occupancy, long chunk_states);
+int pyjac_hip_get_tuning(const pyjac_handle *h, int kernel, int *occupancy, long *chunk_states,
+                         int *waves_per_state);
 
 /*
  * Initial conditions on disk (read_initial_conditions,

@@ -39,8 +39,9 @@ typedef struct {
     long launches;   /* step attempts of the batch (each state one per attempt) */
 } pyjac_integrate_stats;
 
-/* An integrator for one mechanism (PYJAC_ERR_UNSUPPORTED when a state's
-   (NN x NN) iteration matrix does not fit the CU's LDS, NN > 120). */
+/* An integrator for one mechanism, NN up to 512 (PYJAC_ERR_UNSUPPORTED
+   beyond).  Up to NN = 120 a state's (NN x NN) iteration matrix is factorised
+   in LDS; above, in device memory, in place of its Jacobian. */
 int pyjac_hip_integrator_create(const pyjac_mech_tables *tables, int conp,
                                 pyjac_integrator **out);
 int pyjac_hip_integrator_destroy(pyjac_integrator *it);

@@ -107,9 +107,13 @@ size_t block_lds_bytes(const DevMech& m, bool jac, int n_pairs, bool dense = tru
 constexpr int GATHER_MAX = 4;
 
 // one workgroup of 64 * wps threads per state
+// occ: the occupancy target of the instantiation (0: the default; which
+// (wps, occ) pairs exist: rates_block_has / jacobian_block_has)
 hipError_t launch_rates_block(const DevMech& m, const BlockArgs* args, const BlockLds& L, int wps,
                               int conp, char order, long n, const double* param,
-                              const double* phi, double* dphi, hipStream_t stream);
+                              const double* phi, double* dphi, hipStream_t stream, int occ = 0);
+bool rates_block_has(int wps, int occ);
+bool jacobian_block_has(int wps, int occ);
 
 // Second stream and events of the pipelined F-order path
 struct FPipeline {
@@ -125,7 +129,7 @@ hipError_t launch_jacobian_block(const DevMech& m, const BlockArgs* args, const 
                                  int wps, int conp, char order, long n, const double* param,
                                  const double* phi, double* jac, double* chunk_buf,
                                  long chunk_states, hipStream_t stream,
-                                 const FPipeline* pipe = nullptr);
+                                 const FPipeline* pipe = nullptr, int occ = 0);
 
 // one workgroup of 64 * wps threads per state, Jacobian gathered in device
 // memory, chunk_states states (a multiple of 64) per launch: chunk_buf holds

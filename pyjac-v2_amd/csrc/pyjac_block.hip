@@ -1584,8 +1584,12 @@ __device__ __forceinline__ void jacobian_global_state(const BlockArgs* __restric
 #define PJB_GLOB_WAVES 3
 #endif
 
-template <int WPS>
-__global__ void __launch_bounds__(64 * WPS) __attribute__((amdgpu_waves_per_eu(PJB_RATES_WAVES)))
+// OCC: the occupancy target (waves per SIMD) the register allocator is held
+// to; the default is the measured best for GRI-Mech 3.0, the other values
+// are instantiated for the autotuner (2 and 4 for the Jacobian kernels with
+// one and two waves per state, 4 and 6 for the one-wave species rates)
+template <int WPS, int OCC = PJB_RATES_WAVES>
+__global__ void __launch_bounds__(64 * WPS) __attribute__((amdgpu_waves_per_eu(OCC)))
 rates_block_kernel(const BlockArgs* __restrict__ A, int conp, long n,
                    const double* __restrict__ param, const double* __restrict__ phi,
                    double* __restrict__ dphi, Layout L) {
@@ -1595,8 +1599,8 @@ rates_block_kernel(const BlockArgs* __restrict__ A, int conp, long n,
     rates_state<WPS>(A, conp, n, s, param, phi, dphi, L, smem);
 }
 
-template <int WPS>
-__global__ void __launch_bounds__(64 * WPS) __attribute__((amdgpu_waves_per_eu(PJB_JAC_WAVES)))
+template <int WPS, int OCC = PJB_JAC_WAVES>
+__global__ void __launch_bounds__(64 * WPS) __attribute__((amdgpu_waves_per_eu(OCC)))
 jacobian_block_kernel(const BlockArgs* __restrict__ A, int conp, long n,
                       const double* __restrict__ param, const double* __restrict__ phi,
                       double* __restrict__ jac, Layout L, int out_C) {
@@ -1859,27 +1863,37 @@ static hipError_t set_lds(K kernel, size_t lds) {
 
 hipError_t launch_rates_block(const DevMech& m, const BlockArgs* args, const BlockLds& BL, int wps,
                               int conp, char order, long n, const double* param,
-                              const double* phi, double* dphi, hipStream_t stream) {
+                              const double* phi, double* dphi, hipStream_t stream, int occ) {
     pjb::Layout L = order == 'F' ? pjb::Layout{1, (size_t)n} : pjb::Layout{(size_t)m.NN, 1};
     size_t lds = (size_t)BL.total * sizeof(double);
     const long grid = 8 * ((n + 7) / 8);
-#define PJB_RATES(W)                                                                      \
-    case W: {                                                                             \
-        hipError_t e = set_lds(pjb::rates_block_kernel<W>, lds);                         \
+    if (!occ) occ = PJB_RATES_WAVES;
+#define PJB_RATES(W, O)                                                                   \
+    if (wps == W && occ == O) {                                                           \
+        hipError_t e = set_lds(pjb::rates_block_kernel<W, O>, lds);                      \
         if (e != hipSuccess) return e;                                                    \
-        hipLaunchKernelGGL(pjb::rates_block_kernel<W>, dim3(grid), dim3(64 * W), lds,     \
-                           stream, args, conp, n, param, phi, dphi, L);              \
-        break;                                                                            \
+        hipLaunchKernelGGL((pjb::rates_block_kernel<W, O>), dim3(grid), dim3(64 * W), lds, \
+                           stream, args, conp, n, param, phi, dphi, L);                   \
+        return hipGetLastError();                                                         \
     }
-    switch (wps) {
-        PJB_RATES(1)
-        PJB_RATES(2)
-        PJB_RATES(4)
-        PJB_RATES(8)
-        default: return hipErrorInvalidValue;
-    }
+    PJB_RATES(1, PJB_RATES_WAVES)
+    PJB_RATES(2, PJB_RATES_WAVES)
+    PJB_RATES(4, PJB_RATES_WAVES)
+    PJB_RATES(8, PJB_RATES_WAVES)
+    PJB_RATES(1, 4)
+    PJB_RATES(1, 6)
 #undef PJB_RATES
-    return hipGetLastError();
+    return hipErrorInvalidValue;
+}
+
+bool rates_block_has(int wps, int occ) {
+    return !occ || occ == PJB_RATES_WAVES ? (wps == 1 || wps == 2 || wps == 4 || wps == 8)
+                                          : (wps == 1 && (occ == 4 || occ == 6));
+}
+
+bool jacobian_block_has(int wps, int occ) {
+    return !occ || occ == PJB_JAC_WAVES ? (wps == 1 || wps == 2 || wps == 4 || wps == 8)
+                                        : ((wps == 1 || wps == 2) && (occ == 2 || occ == 4));
 }
 
 // chunk of cnt states' entries (C order) -> F-order output columns s0 ..
@@ -1903,37 +1917,51 @@ static hipError_t launch_entries_to_f(const double* chunk, long cnt, int nout, d
 
 // states s0 .. s0 + cnt - 1 of F-order inputs with n states, entries in C
 // order into out (cnt * nout)
+// the Jacobian block kernel of (wps, occ) over n states (a grid of
+// 8 ceil(n / 8) workgroups)
+static hipError_t launch_jac_kernel(int wps, int occ, size_t lds, long grid, hipStream_t stream,
+                                    const BlockArgs* args, int conp, long n, const double* param,
+                                    const double* phi, double* out, pjb::Layout L, int out_C) {
+    if (!occ) occ = PJB_JAC_WAVES;
+#define PJB_JACK(W, O)                                                                    \
+    if (wps == W && occ == O) {                                                           \
+        hipError_t e = set_lds(pjb::jacobian_block_kernel<W, O>, lds);                   \
+        if (e != hipSuccess) return e;                                                    \
+        hipLaunchKernelGGL((pjb::jacobian_block_kernel<W, O>), dim3(grid), dim3(64 * W),  \
+                           lds, stream, args, conp, n, param, phi, out, L, out_C);        \
+        return hipGetLastError();                                                         \
+    }
+    PJB_JACK(1, PJB_JAC_WAVES)
+    PJB_JACK(2, PJB_JAC_WAVES)
+    PJB_JACK(4, PJB_JAC_WAVES)
+    PJB_JACK(8, PJB_JAC_WAVES)
+    PJB_JACK(1, 2)
+    PJB_JACK(1, 4)
+    PJB_JACK(2, 2)
+    PJB_JACK(2, 4)
+#undef PJB_JACK
+    return hipErrorInvalidValue;
+}
+
+// states s0 .. s0 + cnt - 1 of F-order inputs with n states, entries in C
+// order into out (cnt * nout)
 static hipError_t launch_jacobian_block_chunk(const DevMech& m, const BlockArgs* args,
                                               const BlockLds& BL, int wps, int conp, long n,
                                               long s0, long cnt, const double* param,
                                               const double* phi, double* out,
-                                              hipStream_t stream) {
+                                              hipStream_t stream, int occ) {
     pjb::Layout L{1, (size_t)n};
     size_t lds = (size_t)BL.total * sizeof(double);
     const long grid = 8 * ((cnt + 7) / 8);
-#define PJB_JACC(W)                                                                       \
-    case W: {                                                                             \
-        hipError_t e = set_lds(pjb::jacobian_block_kernel<W>, lds);                      \
-        if (e != hipSuccess) return e;                                                    \
-        hipLaunchKernelGGL(pjb::jacobian_block_kernel<W>, dim3(grid), dim3(64 * W), lds,  \
-                           stream, args, conp, cnt, param + s0, phi + s0, out, L, 1);     \
-        break;                                                                            \
-    }
-    switch (wps) {
-        PJB_JACC(1)
-        PJB_JACC(2)
-        PJB_JACC(4)
-        PJB_JACC(8)
-        default: return hipErrorInvalidValue;
-    }
-#undef PJB_JACC
-    return hipGetLastError();
+    return launch_jac_kernel(wps, occ, lds, grid, stream, args, conp, cnt, param + s0, phi + s0,
+                             out, L, 1);
 }
 
 hipError_t launch_jacobian_block(const DevMech& m, const BlockArgs* args, const BlockLds& BL,
                                  int wps, int conp, char order, long n, const double* param,
                                  const double* phi, double* jac, double* chunk_buf,
-                                 long chunk_states, hipStream_t stream, const FPipeline* pipe) {
+                                 long chunk_states, hipStream_t stream, const FPipeline* pipe,
+                                 int occ) {
     if (order == 'F' && chunk_buf && pipe && n > chunk_states) {
         // two chunk buffers: the transpose of chunk k (on the second stream)
         // runs while chunk k + 1's Jacobians are computed; a buffer is
@@ -1950,7 +1978,7 @@ hipError_t launch_jacobian_block(const DevMech& m, const BlockArgs* args, const 
             if (k >= 2) e = hipStreamWaitEvent(stream, pipe->transposed[b], 0);
             if (e == hipSuccess)
                 e = launch_jacobian_block_chunk(m, args, BL, wps, conp, n, s0, cnt, param, phi,
-                                                buf, stream);
+                                                buf, stream, occ);
             if (e == hipSuccess) e = hipEventRecord(pipe->computed[b], stream);
             if (e == hipSuccess) e = hipStreamWaitEvent(pipe->aux, pipe->computed[b], 0);
             if (e == hipSuccess) e = launch_entries_to_f(buf, cnt, nout, jac, n, s0, pipe->aux, true);
@@ -1967,7 +1995,7 @@ hipError_t launch_jacobian_block(const DevMech& m, const BlockArgs* args, const 
         for (long s0 = 0; s0 < n; s0 += chunk_states) {
             const long cnt = std::min(chunk_states, n - s0);
             hipError_t e = launch_jacobian_block_chunk(m, args, BL, wps, conp, n, s0, cnt, param,
-                                                       phi, chunk_buf, stream);
+                                                       phi, chunk_buf, stream, occ);
             if (e != hipSuccess) return e;
             e = launch_entries_to_f(chunk_buf, cnt, nout, jac, n, s0, stream);
             if (e != hipSuccess) return e;
@@ -1977,24 +2005,8 @@ hipError_t launch_jacobian_block(const DevMech& m, const BlockArgs* args, const 
     pjb::Layout L = order == 'F' ? pjb::Layout{1, (size_t)n} : pjb::Layout{(size_t)m.NN, 1};
     size_t lds = (size_t)BL.total * sizeof(double);
     const long grid = 8 * ((n + 7) / 8);
-    const int out_C = order == 'C';
-#define PJB_JAC(W)                                                                        \
-    case W: {                                                                             \
-        hipError_t e = set_lds(pjb::jacobian_block_kernel<W>, lds);                      \
-        if (e != hipSuccess) return e;                                                    \
-        hipLaunchKernelGGL(pjb::jacobian_block_kernel<W>, dim3(grid), dim3(64 * W), lds,  \
-                           stream, args, conp, n, param, phi, jac, L, out_C);        \
-        break;                                                                            \
-    }
-    switch (wps) {
-        PJB_JAC(1)
-        PJB_JAC(2)
-        PJB_JAC(4)
-        PJB_JAC(8)
-        default: return hipErrorInvalidValue;
-    }
-#undef PJB_JAC
-    return hipGetLastError();
+    return launch_jac_kernel(wps, occ, lds, grid, stream, args, conp, n, param, phi, jac, L,
+                             order == 'C');
 }
 
 hipError_t launch_jacobian_global(const DevMech& m, const BlockArgs* args, const BlockLds& BL,

@@ -14,12 +14,18 @@
 // and packs the active ones into the other copy, so that the later attempts
 // -- often many more for the stiffest few states -- launch only over the
 // states still integrating.
+//
+// The iteration matrix I - gamma h J lives in LDS for NN up to 120; for the
+// larger mechanisms (up to 512) it is formed and factorised in device memory,
+// in place of the state's Jacobian, by the same workgroup, and the solves
+// hold ceil(NN / 64) rows per lane.
 
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <vector>
 
 #include "../../include/pyjac_integrator.h"
@@ -27,7 +33,8 @@
 namespace pjros {
 
 constexpr int NT = 256;        // threads per state
-constexpr int NMAX = 120;      // largest NN (the iteration matrix in LDS)
+constexpr int NMAX = 120;      // largest NN with the iteration matrix in LDS
+constexpr int NGMAX = 512;     // largest NN with the iteration matrix in device memory
 constexpr double GAMMA = 1.7071067811865475244;  // 1 + 1/sqrt(2)
 
 enum Status { ACTIVE = 0, DONE = 1, FAILED = 2 };
@@ -111,6 +118,152 @@ __device__ void lu_solve_wave(const double* A, const int* piv, int N, int lane, 
     }
 }
 
+// The same solve for N <= 64 R, lane l holding rows l + 64 r (r < R), for
+// the iteration matrix of the larger mechanisms (in device memory)
+template <int R>
+__device__ void lu_solve_wave_r(const double* A, const int* piv, int N, int lane, double (&b)[R]) {
+    auto get = [&](int row) {
+        double v = 0.0;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const double x = __shfl(b[r], row & 63);
+            if ((row >> 6) == r) v = x;
+        }
+        return v;
+    };
+    auto put = [&](int row, double v) {
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            if (lane == (row & 63) && (row >> 6) == r) b[r] = v;
+    };
+    for (int k = 0; k < N; ++k) {
+        const int p = piv[k];
+        if (p != k) {
+            const double vk = get(k), vp = get(p);
+            put(k, vp);
+            put(p, vk);
+        }
+    }
+    for (int k = 0; k < N; ++k) {  // L y = P b
+        const double xk = get(k);
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int row = lane + 64 * r;
+            if (row > k && row < N) b[r] -= A[(size_t)row * N + k] * xk;
+        }
+    }
+    for (int k = N - 1; k >= 0; --k) {  // U x = y
+        const double xk = get(k) / A[(size_t)k * N + k];
+        put(k, xk);
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int row = lane + 64 * r;
+            if (row < k) b[r] -= A[(size_t)row * N + k] * xk;
+        }
+    }
+}
+
+// Gaussian elimination with partial pivoting of the N x N matrix A (LDS or
+// device memory, this workgroup's own): pv[k] the row exchanged with row k,
+// L (unit, below the diagonal) and U in place
+__device__ void lu_factor_wg(double* A, int* pv, int N, int tid, int lane, int wave) {
+    for (int k = 0; k < N; ++k) {
+        if (wave == 0) {
+            double best = -1.0;
+            int bi = k;
+            for (int i = k + lane; i < N; i += 64) {
+                const double v = fabs(A[(size_t)i * N + k]);
+                if (v > best) { best = v; bi = i; }
+            }
+            for (int o = 32; o > 0; o >>= 1) {
+                const double ob = __shfl_xor(best, o);
+                const int oi = __shfl_xor(bi, o);
+                if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
+            }
+            if (lane == 0) pv[k] = bi;
+        }
+        __syncthreads();
+        const int p = pv[k];
+        if (p != k)
+            for (int j = tid; j < N; j += NT) {
+                const double tmp = A[(size_t)k * N + j];
+                A[(size_t)k * N + j] = A[(size_t)p * N + j];
+                A[(size_t)p * N + j] = tmp;
+            }
+        __syncthreads();
+        const double pivot = A[(size_t)k * N + k];
+        for (int i = k + 1 + tid; i < N; i += NT) A[(size_t)i * N + k] /= pivot;
+        __syncthreads();
+        const int m = N - k - 1;
+        for (int e = tid; e < m * m; e += NT) {
+            const int i = k + 1 + e / m, j = k + 1 + e % m;
+            A[(size_t)i * N + j] -= A[(size_t)i * N + k] * A[(size_t)k * N + j];
+        }
+        __syncthreads();
+    }
+}
+
+// first-step size of a state (h <= 0: 1e-2 of the time the initial rates
+// take to move any component by its error weight), by wave 0
+__device__ void first_step(const Chunk& c, const Params& P, long s, const double* y,
+                           const double* f0, int lane, double* hs_sh) {
+    const int N = c.NN;
+    double h = c.h[s];
+    if (h <= 0.0) {
+        double r = 0.0;
+        for (int i = lane; i < N; i += 64)
+            r = fmax(r, fabs(f0[i]) / (P.atol + P.rtol * fabs(y[i])));
+        r = wave_max_d(r);
+        h = r > 0.0 ? 1e-2 / r : P.t_end;
+        h = fmin(h, P.t_end - c.t[s]);
+        if (lane == 0) c.h[s] = h;
+    }
+    if (lane == 0) *hs_sh = h;
+}
+
+// Mechanisms beyond NMAX: I - gamma h J formed in place of J in device
+// memory, factorised there (this workgroup's own rows, visible to it after
+// each barrier), k1 and the stage state; R rows per lane in the solve
+template <int R>
+__global__ void __launch_bounds__(NT) ros2_stage1_global_kernel(Chunk c, Params P) {
+    __shared__ int pv[NGMAX];
+    __shared__ double hs_sh;
+    const long s = blockIdx.x;
+    if (s >= c.n) return;
+    const int N = c.NN, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const double* y = c.y + s * N;
+    double* y1 = c.y1 + s * N;
+    if (c.status[s] != ACTIVE) {
+        for (int i = tid; i < N; i += NT) y1[i] = y[i];
+        return;
+    }
+    const double* f0 = c.f0 + s * N;
+    if (wave == 0) first_step(c, P, s, y, f0, lane, &hs_sh);
+    __syncthreads();
+    const double gh = -GAMMA * hs_sh;
+    double* A = c.J + s * (size_t)N * N;
+    for (int e = tid; e < N * N; e += NT) {
+        const int i = e / N, j = e - i * N;
+        A[e] = gh * A[e] + (i == j ? 1.0 : 0.0);
+    }
+    __syncthreads();
+    lu_factor_wg(A, pv, N, tid, lane, wave);
+    for (int i = tid; i < N; i += NT) c.piv[s * N + i] = pv[i];
+    if (wave == 0) {
+        double b[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) b[r] = lane + 64 * r < N ? f0[lane + 64 * r] : 0.0;
+        lu_solve_wave_r<R>(A, pv, N, lane, b);
+        const double h = hs_sh;
+        double* k1 = c.k1 + s * N;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int row = lane + 64 * r;
+            if (row < N) { k1[row] = b[r]; y1[row] = y[row] + h * b[r]; }
+        }
+    }
+}
+
 // I - gamma h J -> LU (partial pivoting) in LDS, k1, stage state
 __global__ void __launch_bounds__(NT) ros2_stage1_kernel(Chunk c, Params P) {
     extern __shared__ double A[];  // [N][N]
@@ -126,21 +279,7 @@ __global__ void __launch_bounds__(NT) ros2_stage1_kernel(Chunk c, Params P) {
         return;
     }
     const double* f0 = c.f0 + s * N;
-    if (wave == 0) {
-        double h = c.h[s];
-        if (h <= 0.0) {
-            // first step: 1e-2 of the time the initial rates take to move any
-            // component by its error weight
-            double r = 0.0;
-            for (int i = lane; i < N; i += 64)
-                r = fmax(r, fabs(f0[i]) / (P.atol + P.rtol * fabs(y[i])));
-            r = wave_max_d(r);
-            h = r > 0.0 ? 1e-2 / r : P.t_end;
-            h = fmin(h, P.t_end - c.t[s]);
-            if (lane == 0) c.h[s] = h;
-        }
-        if (lane == 0) hs_sh = h;
-    }
+    if (wave == 0) first_step(c, P, s, y, f0, lane, &hs_sh);
     __syncthreads();
     const double gh = -GAMMA * hs_sh;
     const double* J = c.J + s * (size_t)N * N;
@@ -149,40 +288,7 @@ __global__ void __launch_bounds__(NT) ros2_stage1_kernel(Chunk c, Params P) {
         A[e] = gh * J[e] + (i == j ? 1.0 : 0.0);
     }
     __syncthreads();
-    for (int k = 0; k < N; ++k) {
-        if (wave == 0) {
-            double best = -1.0;
-            int bi = k;
-            for (int i = k + lane; i < N; i += 64) {
-                const double v = fabs(A[i * N + k]);
-                if (v > best) { best = v; bi = i; }
-            }
-            for (int o = 32; o > 0; o >>= 1) {
-                const double ob = __shfl_xor(best, o);
-                const int oi = __shfl_xor(bi, o);
-                if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
-            }
-            if (lane == 0) pv[k] = bi;
-        }
-        __syncthreads();
-        const int p = pv[k];
-        if (p != k)
-            for (int j = tid; j < N; j += NT) {
-                const double tmp = A[k * N + j];
-                A[k * N + j] = A[p * N + j];
-                A[p * N + j] = tmp;
-            }
-        __syncthreads();
-        const double pivot = A[k * N + k];
-        for (int i = k + 1 + tid; i < N; i += NT) A[i * N + k] /= pivot;
-        __syncthreads();
-        const int m = N - k - 1;
-        for (int e = tid; e < m * m; e += NT) {
-            const int i = k + 1 + e / m, j = k + 1 + e % m;
-            A[i * N + j] -= A[i * N + k] * A[k * N + j];
-        }
-        __syncthreads();
-    }
+    lu_factor_wg(A, pv, N, tid, lane, wave);
     double* LU = c.J + s * (size_t)N * N;
     for (int e = tid; e < N * N; e += NT) LU[e] = A[e];
     for (int i = tid; i < N; i += NT) c.piv[s * N + i] = pv[i];
@@ -196,6 +302,9 @@ __global__ void __launch_bounds__(NT) ros2_stage1_kernel(Chunk c, Params P) {
         if (lane + 64 < N) { k1[lane + 64] = b1; y1[lane + 64] = y[lane + 64] + h * b1; }
     }
 }
+
+__device__ void step_decision(const Chunk& c, const Params& P, long s, double h, double err,
+                              bool finite, bool accept);
 
 // k2, error estimate, step decision
 __global__ void __launch_bounds__(NT) ros2_stage2_kernel(Chunk c, Params P) {
@@ -238,7 +347,13 @@ __global__ void __launch_bounds__(NT) ros2_stage2_kernel(Chunk c, Params P) {
         if (lane < N) y[lane] = yn0;
         if (lane + 64 < N) y[lane + 64] = yn1;
     }
-    if (lane == 0) {
+    if (lane == 0) step_decision(c, P, s, h, err, finite, accept);
+}
+
+// accept or reject the attempt of state s, the next step size and status
+__device__ void step_decision(const Chunk& c, const Params& P, long s, double h, double err,
+                              bool finite, bool accept) {
+    {
         double t = c.t[s], hn = h;
         if (accept) {
             t += h;
@@ -269,6 +384,51 @@ __global__ void __launch_bounds__(NT) ros2_stage2_kernel(Chunk c, Params P) {
         c.status[s] = st;
         if (st == ACTIVE) atomicAdd(c.active, 1);
     }
+}
+
+// k2 from the factors in device memory, error estimate, step decision
+template <int R>
+__global__ void __launch_bounds__(64) ros2_stage2_global_kernel(Chunk c, Params P) {
+    __shared__ int pv[NGMAX];
+    const long s = blockIdx.x;
+    if (s >= c.n || c.status[s] != ACTIVE) return;
+    const int N = c.NN, lane = threadIdx.x;
+    const double* A = c.J + s * (size_t)N * N;
+    for (int i = lane; i < N; i += 64) pv[i] = c.piv[s * N + i];
+    __syncthreads();
+    const double* k1 = c.k1 + s * N;
+    const double* f1 = c.f1 + s * N;
+    double* y = c.y + s * N;
+    double kv[R], b[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int row = lane + 64 * r;
+        kv[r] = row < N ? k1[row] : 0.0;
+        b[r] = row < N ? f1[row] - 2.0 * kv[r] : 0.0;
+    }
+    lu_solve_wave_r<R>(A, pv, N, lane, b);
+    const double h = c.h[s];
+    double yn[R], sq = 0.0;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int row = lane + 64 * r;
+        yn[r] = 0.0;
+        if (row < N) {
+            yn[r] = y[row] + 1.5 * h * kv[r] + 0.5 * h * b[r];
+            const double e = 0.5 * h * (kv[r] + b[r]);
+            const double w = P.atol + P.rtol * fmax(fabs(y[row]), fabs(yn[r]));
+            sq += (e / w) * (e / w);
+        }
+    }
+    const double err = sqrt(wave_sum_d(sq) / N);
+    const bool finite = err == err && err < 1e300;
+    const bool accept = P.fixed ? finite : (finite && err <= 1.0);
+    if (accept) {
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            if (lane + 64 * r < N) y[lane + 64 * r] = yn[r];
+    }
+    if (lane == 0) step_decision(c, P, s, h, err, finite, accept);
 }
 
 // the working copy from the caller's states (one workgroup of 64 per state)
@@ -395,7 +555,7 @@ int pyjac_hip_integrator_create(const pyjac_mech_tables* tables, int conp, pyjac
     if (!tables || !out) return PYJAC_ERR_ARG;
     *out = nullptr;
     const int NN = tables->n_species + 1;
-    if (NN > pjros::NMAX) return PYJAC_ERR_UNSUPPORTED;
+    if (NN > pjros::NGMAX) return PYJAC_ERR_UNSUPPORTED;
     pyjac_integrator* it = new pyjac_integrator();
     it->NN = NN;
     int rc = pyjac_hip_create(tables, conp, 'C', PYJAC_JAC_FULL, PYJAC_JAC_EXACT, &it->h);
@@ -442,7 +602,13 @@ int pyjac_hip_integrate_dev(pyjac_integrator* it, size_t n, const double* d_para
         if (hipMalloc(&it->buf, need) != hipSuccess) return PYJAC_ERR_ALLOC;
         it->buf_bytes = need;
     }
-    const size_t lds = (size_t)N * N * sizeof(double);
+    // the iteration matrix in LDS up to NMAX, beyond that in device memory
+    // (in place of the Jacobian) with R = ceil(N / 64) rows per lane
+    // (PYJAC_HIP_INTEGRATOR_DEVICE_LU: the device-memory path for any NN, for
+    // tests that compare it with the LDS path)
+    const bool in_lds = N <= pjros::NMAX && !getenv("PYJAC_HIP_INTEGRATOR_DEVICE_LU");
+    const int R = (N + 63) / 64;
+    const size_t lds = in_lds ? (size_t)N * N * sizeof(double) : 0;
     if (lds > 64 * 1024) {
         if (hipFuncSetAttribute((const void*)pjros::ros2_stage1_kernel,
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess ||
@@ -493,13 +659,41 @@ int pyjac_hip_integrate_dev(pyjac_integrator* it, size_t n, const double* d_para
             int rc = pyjac_hip_species_rates_dev(it->h, m, c.param, c.y, c.f0, s);
             if (!rc) rc = pyjac_hip_jacobian_dev(it->h, m, c.param, c.y, c.J, s);
             if (rc) return rc;
-            hipLaunchKernelGGL(pjros::ros2_stage1_kernel, dim3((unsigned)m), dim3(pjros::NT),
-                               lds, s, c, P);
+            if (in_lds) {
+                hipLaunchKernelGGL(pjros::ros2_stage1_kernel, dim3((unsigned)m), dim3(pjros::NT),
+                                   lds, s, c, P);
+            } else {
+                switch (R) {
+#define PJROS_S1(RR)                                                                      \
+    case RR:                                                                              \
+        hipLaunchKernelGGL(pjros::ros2_stage1_global_kernel<RR>, dim3((unsigned)m),       \
+                           dim3(pjros::NT), 0, s, c, P);                                  \
+        break;
+                    PJROS_S1(1) PJROS_S1(2) PJROS_S1(3) PJROS_S1(4) PJROS_S1(5) PJROS_S1(6) PJROS_S1(7)
+                    PJROS_S1(8)
+#undef PJROS_S1
+                    default: return PYJAC_ERR_UNSUPPORTED;
+                }
+            }
             rc = pyjac_hip_species_rates_dev(it->h, m, c.param, c.y1, c.f1, s);
             if (rc) return rc;
             hipLaunchKernelGGL(pjros::ros2_zero_kernel, dim3(1), dim3(64), 0, s, c.active);
-            hipLaunchKernelGGL(pjros::ros2_stage2_kernel, dim3((unsigned)m), dim3(pjros::NT),
-                               lds, s, c, P);
+            if (in_lds) {
+                hipLaunchKernelGGL(pjros::ros2_stage2_kernel, dim3((unsigned)m), dim3(pjros::NT),
+                                   lds, s, c, P);
+            } else {
+                switch (R) {
+#define PJROS_S2(RR)                                                                      \
+    case RR:                                                                              \
+        hipLaunchKernelGGL(pjros::ros2_stage2_global_kernel<RR>, dim3((unsigned)m),       \
+                           dim3(64), 0, s, c, P);                                         \
+        break;
+                    PJROS_S2(1) PJROS_S2(2) PJROS_S2(3) PJROS_S2(4) PJROS_S2(5) PJROS_S2(6) PJROS_S2(7)
+                    PJROS_S2(8)
+#undef PJROS_S2
+                    default: return PYJAC_ERR_UNSUPPORTED;
+                }
+            }
             if (hipGetLastError() != hipSuccess) return PYJAC_ERR_HIP;
             ++st.launches;
             if (hipMemcpyAsync(it->h_active, c.active, sizeof(int), hipMemcpyDeviceToHost, s) !=

@@ -107,6 +107,10 @@ struct pyjac_handle {
     size_t staged_states = 0;
     int grid_rates = 0, grid_jac = 0, block = 256;
     int variant[4] = {0, 0, 0, 0};
+    // tuned occupancy targets of the block kernels (0: the default build)
+    // and the F-order chunk of the Jacobian (0: the default size)
+    int occ[2] = {0, 0};
+    long fchunk_pref = 0;
     hipStream_t stream = nullptr;
     // names of the tables (version 2), for the introspection entry points
     std::vector<std::string> species_names, reaction_strings;
@@ -210,6 +214,7 @@ long fchunk_want(const pyjac_handle* h, long n) {
     // 1M states against 50.5 ms with direct F-order writes)
     const size_t per = (size_t)h->m.jac_nnz * sizeof(double);
     long want = std::min(65536L, (long)(((size_t)1 << 30) / per) / 64 * 64);
+    if (h->fchunk_pref > 0) want = h->fchunk_pref;
     const char* env = getenv("PYJAC_HIP_FCHUNK");
     if (env) want = atol(env) / 64 * 64;
     if (want <= 0) return 0;
@@ -373,7 +378,8 @@ int run_rates(pyjac_handle* h, long n, const double* param, const double* phi, d
     if (n <= 0) return PYJAC_OK;
     if (use_block(h, 0)) {
         HIP_TRY(pjk_host::launch_rates_block(h->m, h->d_args[0], h->lds[0], block_wps(h, 0),
-                                             h->conp, h->order, n, param, phi, dphi, s));
+                                             h->conp, h->order, n, param, phi, dphi, s,
+                                             h->occ[0]));
         return PYJAC_OK;
     }
     if (h->wave_ok_rates && h->variant[0] == 2) {
@@ -397,7 +403,7 @@ int run_rates_c(pyjac_handle* h, long n, const double* param, const double* phi,
                 hipStream_t s) {
     if (use_block(h, 0)) {
         HIP_TRY(pjk_host::launch_rates_block(h->m, h->d_args[0], h->lds[0], block_wps(h, 0),
-                                             h->conp, 'C', n, param, phi, dphi, s));
+                                             h->conp, 'C', n, param, phi, dphi, s, h->occ[0]));
         return PYJAC_OK;
     }
     size_t per = pjk_host::ws_doubles_per_state(h->m, false);
@@ -449,7 +455,8 @@ int run_jac(pyjac_handle* h, long n, const double* param, const double* phi, dou
                                                 h->conp, h->order, n, param, phi, jac,
                                                 chunked ? h->d_fchunk : nullptr,
                                                 chunked ? h->fchunk_states : 0, s,
-                                                chunked && h->fpipe_ok ? &h->fpipe : nullptr));
+                                                chunked && h->fpipe_ok ? &h->fpipe : nullptr,
+                                                h->occ[1]));
         return PYJAC_OK;
     }
     if (use_glob(h)) {
@@ -1260,7 +1267,8 @@ int pyjac_hip_jacobian_dev_noalloc(pyjac_handle* h, size_t n, const double* para
         if (h->order == 'C') {
             HIP_TRY(pjk_host::launch_jacobian_block(h->m, h->d_args[1], h->lds[1],
                                                     block_wps(h, 1), h->conp, 'C', (long)n,
-                                                    param, phi, jac, nullptr, 0, s));
+                                                    param, phi, jac, nullptr, 0, s, nullptr,
+                                                    h->occ[1]));
             return PYJAC_OK;
         }
         // F order: C-order chunks of whole multiples of 64 states
@@ -1273,7 +1281,7 @@ int pyjac_hip_jacobian_dev_noalloc(pyjac_handle* h, size_t n, const double* para
         HIP_TRY(pjk_host::noalloc_scratch(&buf));
         HIP_TRY(pjk_host::launch_jacobian_block(h->m, h->d_args[1], h->lds[1], block_wps(h, 1),
                                                 h->conp, 'F', (long)n, param, phi, jac, buf, cs,
-                                                s, nullptr));
+                                                s, nullptr, h->occ[1]));
         return PYJAC_OK;
     }
     if (use_glob(h)) {
@@ -1572,41 +1580,72 @@ int pyjac_hip_autotune(pyjac_handle* h, int kernel, size_t n, const double* d_pa
     if (!h || (kernel != 0 && kernel != 1) || !n || !d_param || !d_phi || !d_out)
         return fail(PYJAC_ERR_ARG, "bad autotune call");
     hipStream_t s = (hipStream_t)stream;
-    std::vector<int> cand;
+    // candidates: (variant, occupancy target); then, for the F-order LDS
+    // Jacobian, the chunk size of the winner
+    struct Cand { int v, occ; };
+    std::vector<Cand> cand;
     bool block = kernel == 0 ? h->block_ok_rates : h->block_ok_jac;
     bool wave = kernel == 0 ? h->wave_ok_rates : h->wave_ok_jac;
     if (block)
-        for (int w : {1, 2, 4, 8}) cand.push_back(10 + w);
+        for (int w : {1, 2, 4, 8}) {
+            cand.push_back(Cand{10 + w, 0});
+            for (int occ : {2, 4, 6}) {
+                const bool has = kernel == 0 ? pjk_host::rates_block_has(w, occ)
+                                             : pjk_host::jacobian_block_has(w, occ);
+                if (has) cand.push_back(Cand{10 + w, occ});
+            }
+        }
     if (kernel == 1 && h->glob_ok_jac)
-        for (int w : {2, 4, 8}) cand.push_back(20 + w);
-    if (wave) cand.push_back(2);
-    if (!(kernel == 1 && h->bt.approx_sums)) cand.push_back(1);
+        for (int w : {2, 4, 8}) cand.push_back(Cand{20 + w, 0});
+    if (wave) cand.push_back(Cand{2, 0});
+    if (!(kernel == 1 && h->bt.approx_sums)) cand.push_back(Cand{1, 0});
     const int saved_variant = h->variant[kernel], saved_wps = h->wps[kernel];
-    const int saved_wps_g = h->wps_g;
+    const int saved_wps_g = h->wps_g, saved_occ = h->occ[kernel];
+    const long saved_chunk = h->fchunk_pref;
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
-    int best = -1;
-    float best_t = 0.0f;
-    int rc = PYJAC_OK;
-    for (int v : cand) {
-        h->variant[kernel] = v;
-        auto run = [&]() {
-            return kernel == 0 ? run_rates(h, (long)n, d_param, d_phi, d_out, s)
-                               : run_jac(h, (long)n, d_param, d_phi, d_out, s);
-        };
-        if ((rc = run())) break;
-        if (hipEventRecord(e0, s) != hipSuccess) { rc = fail(PYJAC_ERR_HIP, "event"); break; }
+    auto run = [&]() {
+        return kernel == 0 ? run_rates(h, (long)n, d_param, d_phi, d_out, s)
+                           : run_jac(h, (long)n, d_param, d_phi, d_out, s);
+    };
+    // one warm-up call, then the mean of three
+    auto timed = [&](float* ms) {
+        int rc = run();
+        if (rc) return rc;
+        if (hipEventRecord(e0, s) != hipSuccess) return fail(PYJAC_ERR_HIP, "event");
         for (int r = 0; r < 3 && !rc; ++r) rc = run();
-        if (rc) break;
-        if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
-            rc = fail(PYJAC_ERR_HIP, "event");
-            break;
-        }
+        if (rc) return rc;
+        if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess)
+            return fail(PYJAC_ERR_HIP, "event");
+        (void)hipEventElapsedTime(ms, e0, e1);
+        *ms /= 3.0f;
+        return PYJAC_OK;
+    };
+    Cand best{-1, 0};
+    float best_t = 0.0f;
+    long best_chunk = 0;
+    int rc = PYJAC_OK;
+    for (const Cand& c : cand) {
+        h->variant[kernel] = c.v;
+        h->occ[kernel] = c.occ;
         float ms = 0.0f;
-        (void)hipEventElapsedTime(&ms, e0, e1);
-        ms /= 3.0f;
-        if (best < 0 || ms < best_t) { best = v; best_t = ms; }
+        if ((rc = timed(&ms))) break;
+        if (best.v < 0 || ms < best_t) { best = c; best_t = ms; }
+    }
+    // the F-order chunk of the LDS Jacobian: a few sizes around the default
+    if (!rc && kernel == 1 && best.v > 10 && best.v < 20 && h->order == 'F' &&
+        !getenv("PYJAC_HIP_FCHUNK")) {
+        h->variant[kernel] = best.v;
+        h->occ[kernel] = best.occ;
+        for (long cs : {16384L, 32768L, 131072L}) {
+            if (cs >= (long)n) continue;
+            h->fchunk_pref = cs;
+            float ms = 0.0f;
+            if ((rc = timed(&ms))) break;
+            if (ms < best_t) { best_t = ms; best_chunk = cs; }
+        }
+        h->fchunk_pref = saved_chunk;
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
@@ -1614,20 +1653,51 @@ int pyjac_hip_autotune(pyjac_handle* h, int kernel, size_t n, const double* d_pa
         h->variant[kernel] = saved_variant;
         h->wps[kernel] = saved_wps;
         h->wps_g = saved_wps_g;
+        h->occ[kernel] = saved_occ;
         return rc;
     }
-    if (best > 20) {
+    h->occ[kernel] = best.occ;
+    if (kernel == 1) h->fchunk_pref = best_chunk;
+    if (best.v > 20) {
         // automatic choice only when the dense accumulator does not fit
-        h->variant[kernel] = h->block_ok_jac ? best : 0;
-        h->wps_g = best - 20;
-    } else if (best > 10) {
+        h->variant[kernel] = h->block_ok_jac ? best.v : 0;
+        h->wps_g = best.v - 20;
+    } else if (best.v > 10) {
         h->variant[kernel] = 0;
-        h->wps[kernel] = best - 10;
+        h->wps[kernel] = best.v - 10;
     } else {
-        h->variant[kernel] = best;
+        h->variant[kernel] = best.v;
     }
-    if (best_variant) *best_variant = best;
+    if (best_variant) *best_variant = best.v;
     if (best_ms) *best_ms = best_t;
+    return PYJAC_OK;
+}
+
+int pyjac_hip_set_tuning(pyjac_handle* h, int kernel, int occupancy, long chunk_states) {
+    if (!h || (kernel != 0 && kernel != 1) || occupancy < 0 || chunk_states < 0)
+        return fail(PYJAC_ERR_ARG, "bad tuning");
+    if (occupancy) {
+        const int w = block_wps(h, kernel);
+        const bool has = kernel == 0 ? pjk_host::rates_block_has(w, occupancy)
+                                     : pjk_host::jacobian_block_has(w, occupancy);
+        if (!has)
+            return fail(PYJAC_ERR_UNSUPPORTED,
+                        "no build of the %d-wave block kernel for occupancy %d", w, occupancy);
+    }
+    if (chunk_states && (kernel != 1 || chunk_states % 64))
+        return fail(PYJAC_ERR_ARG, "chunk sizes are multiples of 64, for the Jacobian");
+    h->occ[kernel] = occupancy;
+    if (kernel == 1) h->fchunk_pref = chunk_states;
+    return PYJAC_OK;
+}
+
+int pyjac_hip_get_tuning(const pyjac_handle* h, int kernel, int* occupancy, long* chunk_states,
+                         int* waves_per_state) {
+    if (!h || (kernel != 0 && kernel != 1)) return fail(PYJAC_ERR_ARG, "bad tuning query");
+    if (occupancy) *occupancy = h->occ[kernel];
+    if (chunk_states) *chunk_states = kernel == 1 ? h->fchunk_pref : 0;
+    if (waves_per_state) *waves_per_state = use_glob(h) && kernel == 1 ? glob_wps(h)
+                                                                       : block_wps(h, kernel);
     return PYJAC_OK;
 }
 

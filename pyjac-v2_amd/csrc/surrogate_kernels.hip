@@ -53,9 +53,19 @@ struct EpiArgs {
 
 constexpr int BM = 128, BK = 64, PAD = 8;
 
+// LDS of one (BM x BN) block: two stages of the A and B tiles (BK wide, each
+// row padded by PAD), reused for the bf16 output tile of the epilogue
+template <int BN>
+constexpr size_t gemm_lds_bytes() {
+    constexpr size_t ops = 2 * (size_t)(BM + BN) * (BK + PAD), outs = (size_t)BM * (BN + 8);
+    return (ops > outs ? ops : outs) * sizeof(bf16_t);
+}
+
 // C[M][N] = A[M][K] * B[N][K]^T; M % 128 == 0, N % BN == 0, K % 32 == 0.
-// One LDS buffer; the next K tile is fetched into registers while the
-// current one feeds the MFMAs.
+// Two LDS stages: the next K tile is fetched into registers while the
+// current stage feeds the MFMAs, then stored into the other stage, with one
+// barrier per K tile.  Four waves, each a (BM / WM) x (BN / WN) block of
+// 32 x 32 MFMA tiles.
 template <int BN, int EPI>
 __global__ void __launch_bounds__(256)
 gemm_nt(const bf16_t* __restrict__ A, int lda, const bf16_t* __restrict__ B, int ldb, int M,
@@ -66,13 +76,11 @@ gemm_nt(const bf16_t* __restrict__ A, int lda, const bf16_t* __restrict__ B, int
     constexpr int TN = BN / WN / 32;       // along N
     constexpr int CA = BM * (BK / 8) / 256;  // 16-byte chunks per thread per tile
     constexpr int CB = BN * (BK / 8) / 256;
-    // one LDS buffer: the A and B tiles of the main loop, then the bf16
-    // output tile of the epilogue
     constexpr int LDT = BN + 8;
-    constexpr int OPS = (BM + BN) * (BK + PAD), OUTS = BM * LDT;
-    __shared__ __attribute__((aligned(16))) bf16_t smem[OPS > OUTS ? OPS : OUTS];
-    bf16_t (*sA)[BK + PAD] = (bf16_t (*)[BK + PAD])smem;
-    bf16_t (*sB)[BK + PAD] = (bf16_t (*)[BK + PAD])(smem + BM * (BK + PAD));
+    constexpr int STAGE = (BM + BN) * (BK + PAD);
+    extern __shared__ __attribute__((aligned(16))) bf16_t smem[];
+    auto sA = [&](int st) { return (bf16_t (*)[BK + PAD])(smem + st * STAGE); };
+    auto sB = [&](int st) { return (bf16_t (*)[BK + PAD])(smem + st * STAGE + BM * (BK + PAD)); };
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave / WN, wn = wave % WN;
     const long m0 = (long)blockIdx.y * BM, n0 = (long)blockIdx.x * BN;
@@ -98,36 +106,41 @@ gemm_nt(const bf16_t* __restrict__ A, int lda, const bf16_t* __restrict__ B, int
             rb[c] = (k0 + q < K) ? *(const u32x4*)&B[(n0 + r) * ldb + k0 + q] : u32x4{0, 0, 0, 0};
         }
     };
-    auto stash = [&]() {
+    auto stash = [&](int st) {
+        bf16_t (*a)[BK + PAD] = sA(st);
+        bf16_t (*b)[BK + PAD] = sB(st);
 #pragma unroll
         for (int c = 0; c < CA; ++c) {
             const int ch = tid + c * 256, r = ch / (BK / 8), q = (ch % (BK / 8)) * 8;
-            *(u32x4*)&sA[r][q] = ra[c];
+            *(u32x4*)&a[r][q] = ra[c];
         }
 #pragma unroll
         for (int c = 0; c < CB; ++c) {
             const int ch = tid + c * 256, r = ch / (BK / 8), q = (ch % (BK / 8)) * 8;
-            *(u32x4*)&sB[r][q] = rb[c];
+            *(u32x4*)&b[r][q] = rb[c];
         }
     };
     fetch(0);
-    stash();
+    stash(0);
     __syncthreads();
+    int cur = 0;
     for (int k0 = 0; k0 < K; k0 += BK) {
         const bool more = k0 + BK < K;
         if (more) fetch(k0 + BK);
+        const bf16_t (*a)[BK + PAD] = sA(cur);
+        const bf16_t (*b)[BK + PAD] = sB(cur);
 #pragma unroll
         for (int kk = 0; kk < BK; kk += 16) {
             bf16x8 fa[TM], fb[TN];
 #pragma unroll
             for (int i = 0; i < TM; ++i) {
                 const int r = wm * (TM * 32) + i * 32 + (lane & 31);
-                fa[i] = *(const bf16x8*)&sA[r][kk + (lane >> 5) * 8];
+                fa[i] = *(const bf16x8*)&a[r][kk + (lane >> 5) * 8];
             }
 #pragma unroll
             for (int j = 0; j < TN; ++j) {
                 const int r = wn * (TN * 32) + j * 32 + (lane & 31);
-                fb[j] = *(const bf16x8*)&sB[r][kk + (lane >> 5) * 8];
+                fb[j] = *(const bf16x8*)&b[r][kk + (lane >> 5) * 8];
             }
 #pragma unroll
             for (int i = 0; i < TM; ++i)
@@ -135,16 +148,15 @@ gemm_nt(const bf16_t* __restrict__ A, int lda, const bf16_t* __restrict__ B, int
                 for (int j = 0; j < TN; ++j)
                     acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[i], fb[j], acc[i][j], 0, 0, 0);
         }
+        // the other stage was last read before the previous barrier
+        if (more) stash(cur ^ 1);
         __syncthreads();
-        if (more) {
-            stash();
-            __syncthreads();
-        }
+        cur ^= 1;
     }
     // epilogue.  fp32 results are stored directly (32 lanes write 128
     // contiguous bytes of a row); bf16 results go through LDS so that the
     // row-major tile and its transpose leave in 16-byte stores.
-    bf16_t* tile = smem;  // the operand tiles are dead after the last barrier
+    bf16_t* tile = smem;  // the operand stages are dead after the last barrier
 #pragma unroll
     for (int i = 0; i < TM; ++i)
 #pragma unroll
@@ -450,8 +462,17 @@ static int launch_gemm(int bn, int epi, const bf16_t* A, int lda, const bf16_t* 
     dim3 grid(N / bn, M / sgk::BM), block(256);
 #define SGK_G(BN_, E_)                                                                        \
     if (bn == BN_ && epi == E_) {                                                             \
-        hipLaunchKernelGGL((sgk::gemm_nt<BN_, E_>), grid, block, 0, st, A, lda, B, ldb, M, N, \
-                           K, e);                                                            \
+        constexpr size_t lds = sgk::gemm_lds_bytes<BN_>();                                    \
+        static bool attr_##BN_##_##E_ = false;                                                \
+        if (!attr_##BN_##_##E_) {                                                             \
+            if (hipFuncSetAttribute((const void*)sgk::gemm_nt<BN_, E_>,                       \
+                                    hipFuncAttributeMaxDynamicSharedMemorySize,               \
+                                    (int)lds) != hipSuccess)                                  \
+                return 2;                                                                     \
+            attr_##BN_##_##E_ = true;                                                         \
+        }                                                                                     \
+        hipLaunchKernelGGL((sgk::gemm_nt<BN_, E_>), grid, block, lds, st, A, lda, B, ldb, M,  \
+                           N, K, e);                                                          \
         return hipGetLastError() == hipSuccess ? 0 : 2;                                       \
     }
     SGK_G(128, 0) SGK_G(128, 1) SGK_G(128, 2) SGK_G(128, 3)

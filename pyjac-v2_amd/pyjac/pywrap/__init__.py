@@ -78,6 +78,9 @@ def load_library(path=None):
     L.pyjac_hip_autotune.argtypes = [P, I, S, P, P, P, P, ctypes.POINTER(I),
                                      ctypes.POINTER(ctypes.c_float)]
     L.pyjac_hip_set_fd_options.argtypes = [P, I, I, ctypes.c_double, ctypes.c_double]
+    L.pyjac_hip_set_tuning.argtypes = [P, I, I, ctypes.c_long]
+    L.pyjac_hip_get_tuning.argtypes = [P, I, ctypes.POINTER(I), ctypes.POINTER(ctypes.c_long),
+                                       ctypes.POINTER(I)]
     L.pyjac_hip_read_initial_conditions.argtypes = [ctypes.c_char_p, S, I, P, P, ctypes.c_char]
     L.pyjac_hip_write_data.argtypes = [ctypes.c_char_p, P, S]
     if path == LIB_PATH or _lib is None:
@@ -242,9 +245,19 @@ class _KernelBase(object):
         ms = ctypes.c_float()
         check(self.lib.pyjac_hip_autotune(self._h, self._variant_slot, _numel(param), pp, ph, po,
                                           _current_stream_ptr(), ctypes.byref(v), ctypes.byref(ms)))
+        occ, chunk, _ = self.tuning()
         if cache:
-            _tune_store(self._tune_key(), int(v.value), float(ms.value))
+            _tune_store(self._tune_key(), int(v.value), float(ms.value), occ, chunk)
         return int(v.value), float(ms.value)
+
+    def tuning(self):
+        """(occupancy target, F-order chunk states, waves per state) in use
+        (0: the default build / size)."""
+        occ, ws = ctypes.c_int(), ctypes.c_int()
+        chunk = ctypes.c_long()
+        check(self.lib.pyjac_hip_get_tuning(self._h, self._variant_slot, ctypes.byref(occ),
+                                            ctypes.byref(chunk), ctypes.byref(ws)))
+        return int(occ.value), int(chunk.value), int(ws.value)
 
     def _tune_key(self):
         import hashlib
@@ -263,6 +276,9 @@ class _KernelBase(object):
         if choice is not None:
             # a cached choice this build cannot run keeps the automatic one
             self.lib.pyjac_hip_set_variant(self._h, self._variant_slot, int(choice['variant']))
+            self.lib.pyjac_hip_set_tuning(self._h, self._variant_slot,
+                                          int(choice.get('occupancy', 0)),
+                                          int(choice.get('chunk', 0)))
 
     def _check_sizes(self, param, phi, out):
         """n from param; phi must hold n * NN values and out n * (width of
@@ -308,10 +324,10 @@ def _tune_load():
         return {}
 
 
-def _tune_store(key, variant, ms):
+def _tune_store(key, variant, ms, occupancy=0, chunk=0):
     import json
     d = _tune_load()
-    d[key] = {'variant': variant, 'ms': ms}
+    d[key] = {'variant': variant, 'ms': ms, 'occupancy': occupancy, 'chunk': chunk}
     path = _tune_path()
     os.makedirs(os.path.dirname(path), exist_ok=True)
     tmp = path + '.%d.tmp' % os.getpid()

@@ -99,6 +99,10 @@ def test_graph_replays_follow_new_inputs(mechs):
 
 
 def test_autotune_keeps_results_right(mechs, tmp_path, monkeypatch):
+    """The autotuner's choice (kernel family, waves per state, occupancy
+    target and, for the F-order Jacobian, the chunk size) is cached, picked
+    up by a new kernel of the same key, and gives results within the parity
+    bounds."""
     monkeypatch.setenv('PYJAC_HIP_TUNE_CACHE', str(tmp_path / 'tune.json'))
     mech = mechs('gri30')
     n = 16384
@@ -109,7 +113,10 @@ def test_autotune_keeps_results_right(mechs, tmp_path, monkeypatch):
         v, ms = k.autotune(p, f, out)
         assert v in (1, 2, 11, 12, 14, 18, 22, 24, 28) and ms > 0
         assert (tmp_path / 'tune.json').exists()
+        tuned = k.tuning()
+        print('autotune', K.__name__, 'variant', v, 'ms', ms, '(occupancy, chunk, waves)', tuned)
         k2 = K(n, 0, mech=mech, conp=False, order='F', jac_format=1)  # picks the tuned choice
+        assert k2.tuning() == tuned
         out2 = torch.empty_like(out)
         k2(p, f, out2)
         torch.cuda.synchronize()

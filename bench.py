@@ -148,11 +148,16 @@ def main():
     d_dphi = torch.empty_like(d_phi)
     d_jac = torch.empty((mech.nnz, n), dtype=torch.float64, device=dev)
     jk = PyJacobianKernel(n, 0, mech=mech, conp=False, order='F', jac_format=1)
-    rk = PySpeciesRatesKernel(n, 0, mech=mech, conp=False, order='F')
+    # species rates by the mechanism-specialised module when it is built
+    # (__graft_entry__.build()), else by the block kernel; the block
+    # kernel is timed beside it either way
+    rk = PySpeciesRatesKernel(n, 0, mech=mech, conp=False, order='F', module='auto')
+    rk_block = PySpeciesRatesKernel(n, 0, mech=mech, conp=False, order='F')
 
     for _ in range(args.warmup):
         jk(d_param, d_phi, d_jac)
         rk(d_param, d_phi, d_dphi)
+        rk_block(d_param, d_phi, d_dphi)
     torch.cuda.synchronize()
 
     stream = torch.cuda.current_stream()
@@ -170,6 +175,14 @@ def main():
     wall = time.perf_counter() - t0
     jac_ms = float(np.mean([a.elapsed_time(b) for a, b, _ in ev]))
     rates_ms = float(np.mean([b.elapsed_time(c) for _, b, c in ev]))
+    # the block kernel's species rates, outside the timed steps
+    b0, b1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    b0.record(stream)
+    for _ in range(args.steps):
+        rk_block(d_param, d_phi, d_dphi)
+    b1.record(stream)
+    torch.cuda.synchronize()
+    rates_block_ms = b0.elapsed_time(b1) / args.steps
 
     jac_states_s = n / (jac_ms * 1e-3)
     rates_states_s = n / (rates_ms * 1e-3)
@@ -195,8 +208,12 @@ def main():
                    'states': n, 'conp': False, 'order': 'F',
                    'jac_format': 'sparse CCS', 'jac_nnz': mech.nnz},
         'species_rates_states_per_s': rates_states_s,
+        'species_rates_kernel': ('pyjac_lane_rates (mechanism-specialised module, one state '
+                                 'per lane)' if rk.module else 'rates_block_kernel<1>'),
         'jacobian_ms': jac_ms,
         'species_rates_ms': rates_ms,
+        'species_rates_block_kernel_ms': rates_block_ms,
+        'species_rates_block_kernel_states_per_s': n / (rates_block_ms * 1e-3),
         'roofline': {'bound': 'hbm', 'achieved': achieved, 'peak': HBM_PEAK_GBS,
                      'unit': 'GB/s', 'frac': achieved / HBM_PEAK_GBS,
                      'traffic': measured_traffic(n),

@@ -315,10 +315,30 @@ size_t pyjac_hip_required_memory(const pyjac_handle *h);
    10 + W block-per-state kernels with W = 1, 2, 4 or 8 waves per state,
    20 + W (Jacobian) block-per-state kernels that gather the Jacobian in
    device memory (the automatic choice when the dense accumulator does not
-   fit in LDS).  PYJAC_ERR_UNSUPPORTED when this mechanism cannot run the
-   variant (its arrays exceed the variant's LDS). */
+   fit in LDS), 30 (species rates) the mechanism-specialised module loaded
+   by pyjac_hip_load_module (the automatic choice once loaded).
+   PYJAC_ERR_UNSUPPORTED when this mechanism cannot run the variant (its
+   arrays exceed the variant's LDS, or no module is loaded). */
 int pyjac_hip_set_variant(pyjac_handle *h, int kernel, int variant);
 int pyjac_hip_get_variant(const pyjac_handle *h, int kernel);
+
+/*
+ * Mechanism-specialised species rates.  The reference generates its kernels
+ * per mechanism with the rate parameters as literals (libgen compiles them,
+ * pyjac/libgen/libgen.py:24-283; examples/opencl/ch4/species_rates.oclh);
+ * pyjac.libgen.lanegen does the same for gfx950: a code object whose kernel
+ * evaluates one state per lane in straight-line code.  load_module attaches
+ * such a code object to a handle made from the same tables: it checks the
+ * module's fingerprint (the tables' 64-bit hash, the species and reaction
+ * counts and the module version) and refuses a module generated from other
+ * tables (PYJAC_ERR_ARG) or a file it cannot load (PYJAC_ERR_IO).  Once
+ * loaded, the handle's species rates use it (variant 30, every data order and
+ * both CONP / CONV), the finite-difference Jacobian's perturbed states
+ * included; the analytical Jacobian and chem utils are unchanged.
+ * tables_hash gives the fingerprint of a tables struct.
+ */
+int pyjac_hip_tables_hash(const pyjac_mech_tables *tables, unsigned long long *out);
+int pyjac_hip_load_module(pyjac_handle *h, const char *path);
 
 /*
  * Ragged batches (no counterpart in the reference, which builds one kernel

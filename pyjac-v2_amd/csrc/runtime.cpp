@@ -118,6 +118,11 @@ struct pyjac_handle {
     // entry points and the per-state report; two counters on the device
     int input_check = 0;
     unsigned long long* d_counters = nullptr;
+    // mechanism-specialised species-rate module (variant 30): the code
+    // object generated for these tables, and their fingerprint
+    unsigned long long tables_hash = 0;
+    hipModule_t lane_mod = nullptr;
+    hipFunction_t lane_fn = nullptr;
 };
 
 namespace {
@@ -183,7 +188,14 @@ int block_wps(const pyjac_handle* h, int kernel) {
     return 1;
 }
 
+// the species rates by the loaded specialised module: asked for, or
+// automatic once a module is loaded
+bool use_module(const pyjac_handle* h) {
+    return h->lane_fn && (h->variant[0] == 0 || h->variant[0] == 30);
+}
+
 bool use_block(const pyjac_handle* h, int kernel) {
+    if (kernel == 0 && use_module(h)) return false;
     int v = h->variant[kernel];
     bool ok = kernel == 0 ? h->block_ok_rates : h->block_ok_jac;
     return ok && (v == 0 || (v > 10 && v < 20));
@@ -356,7 +368,7 @@ int ensure_chunk(pyjac_handle* h, long n) {
 int prepare_lane(pyjac_handle* h, int kernel, long n) {
     if (kernel == 1 && h->jac_type == PYJAC_JAC_FINITE_DIFFERENCE) {
         int rc = ensure_fd(h, n);
-        if (rc || use_block(h, 0)) return rc;
+        if (rc || use_block(h, 0) || use_module(h)) return rc;
         // the species-rate launches over the perturbed states
         const long m = fd_chunk(h, n) * h->m.NN * h->fd_K;
         size_t per = pjk_host::ws_doubles_per_state(h->m, false);
@@ -365,7 +377,7 @@ int prepare_lane(pyjac_handle* h, int kernel, long n) {
     }
     if (kernel == 1 && use_glob(h)) return ensure_chunk(h, n);
     if (kernel == 1 && use_block(h, 1)) return ensure_fchunk(h, n);
-    if (use_block(h, kernel)) return PYJAC_OK;
+    if (use_block(h, kernel) || (kernel == 0 && use_module(h))) return PYJAC_OK;
     if (h->variant[kernel] == 2 && (kernel == 0 ? h->wave_ok_rates : h->wave_ok_jac))
         return PYJAC_OK;
     size_t per = pjk_host::ws_doubles_per_state(h->m, kernel == 1);
@@ -373,9 +385,27 @@ int prepare_lane(pyjac_handle* h, int kernel, long n) {
     return ensure_ws(h, per * (size_t)grid * h->block * sizeof(double));
 }
 
+// the specialised module's kernel: one state per lane, 64 per workgroup;
+// entry i of state s at [s * sN + i * sI]
+constexpr int LANE_VERSION = 1;
+constexpr int LANE_BLOCK = 64;
+
+int launch_module(pyjac_handle* h, long n, const double* param, const double* phi, double* dphi,
+                  char order, hipStream_t s) {
+    long sN = order == 'F' ? 1 : h->m.NN, sI = order == 'F' ? n : 1;
+    int conp = h->conp;
+    void* args[] = {&n, &param, &phi, &dphi, &sN, &sI, &conp};
+    const long grid = (n + LANE_BLOCK - 1) / LANE_BLOCK;
+    if (grid > 0x7fffffffL) return fail(PYJAC_ERR_ARG, "too many states for one launch");
+    HIP_TRY(hipModuleLaunchKernel(h->lane_fn, (unsigned)grid, 1, 1, LANE_BLOCK, 1, 1, 0, s, args,
+                                  nullptr));
+    return PYJAC_OK;
+}
+
 int run_rates(pyjac_handle* h, long n, const double* param, const double* phi, double* dphi,
               hipStream_t s) {
     if (n <= 0) return PYJAC_OK;
+    if (use_module(h)) return launch_module(h, n, param, phi, dphi, h->order, s);
     if (use_block(h, 0)) {
         HIP_TRY(pjk_host::launch_rates_block(h->m, h->d_args[0], h->lds[0], block_wps(h, 0),
                                              h->conp, h->order, n, param, phi, dphi, s,
@@ -401,6 +431,7 @@ int run_rates(pyjac_handle* h, long n, const double* param, const double* phi, d
 // Jacobian), whatever the handle's data order
 int run_rates_c(pyjac_handle* h, long n, const double* param, const double* phi, double* dphi,
                 hipStream_t s) {
+    if (use_module(h)) return launch_module(h, n, param, phi, dphi, 'C', s);
     if (use_block(h, 0)) {
         HIP_TRY(pjk_host::launch_rates_block(h->m, h->d_args[0], h->lds[0], block_wps(h, 0),
                                              h->conp, 'C', n, param, phi, dphi, s, h->occ[0]));
@@ -570,6 +601,44 @@ int check_dev_inputs(pyjac_handle* h, size_t n, const double* param, const doubl
         return fail(PYJAC_ERR_ARG, "phi of state %zu is not finite (NaN or infinity; %zu states)",
                     first, bad);
     return PYJAC_OK;
+}
+
+// 64-bit FNV-1a over the species and reaction counts and the tables a
+// specialised species-rate module is generated from, in a fixed order
+// (pyjac/libgen/lanegen.py computes the same number)
+unsigned long long tables_fingerprint(const pyjac_mech_tables* mt) {
+    unsigned long long h = 0xcbf29ce484222325ULL;
+    auto add = [&](const void* p, size_t bytes) {
+        const unsigned char* c = (const unsigned char*)p;
+        for (size_t i = 0; i < bytes; ++i) {
+            h ^= c[i];
+            h *= 0x100000001b3ULL;
+        }
+    };
+    const int32_t head[2] = {mt->n_species, mt->n_reactions};
+    add(head, sizeof(head));
+    const size_t Ns = mt->n_species, Nr = mt->n_reactions, nrs = mt->n_rxn_spec;
+    const size_t nsi = mt->n_simple, nf = mt->n_fall, ntr = mt->n_troe, nsr = mt->n_sri;
+    const size_t nt = mt->n_thd, nts = mt->n_thd_spec, npl = mt->n_plog, nc = mt->n_cheb;
+#define PJ_H(field, count) \
+    if ((count) > 0) add(mt->field, (size_t)(count) * sizeof(*mt->field))
+    PJ_H(rxn_off, Nr + 1); PJ_H(rxn_spec, nrs); PJ_H(nu_prod, nrs); PJ_H(nu_reac, nrs);
+    PJ_H(nu_sum, Nr); PJ_H(rev_mask, Nr); PJ_H(thd_mask, Nr); PJ_H(simple_mask, Nr);
+    PJ_H(plog_mask, Nr); PJ_H(cheb_mask, Nr); PJ_H(fall_mask, Nr);
+    PJ_H(simple_form, nsi); PJ_H(simple_A, nsi); PJ_H(simple_b, nsi); PJ_H(simple_Ta, nsi);
+    PJ_H(fall_type, nf); PJ_H(fall_form, nf); PJ_H(fall_blend, nf); PJ_H(fall_param, nf);
+    PJ_H(fall_A, nf); PJ_H(fall_b, nf); PJ_H(fall_Ta, nf);
+    PJ_H(troe_a, ntr); PJ_H(troe_T1inv, ntr); PJ_H(troe_T2, ntr); PJ_H(troe_T3inv, ntr);
+    PJ_H(sri_a, nsr); PJ_H(sri_b, nsr); PJ_H(sri_c, nsr); PJ_H(sri_d, nsr); PJ_H(sri_e, nsr);
+    PJ_H(thd_map, nt); PJ_H(thd_type, nt); PJ_H(thd_off, nt + 1); PJ_H(thd_spec, nts);
+    PJ_H(thd_eff, nts);
+    PJ_H(plog_np, npl); PJ_H(plog_params, 4 * npl * (size_t)mt->plog_max_p);
+    PJ_H(cheb_nt, nc); PJ_H(cheb_np, nc);
+    PJ_H(cheb_params, nc * (size_t)mt->cheb_max_t * (size_t)mt->cheb_max_p);
+    PJ_H(cheb_plim, 2 * nc); PJ_H(cheb_tlim, 2 * nc);
+    PJ_H(a_lo, 7 * Ns); PJ_H(a_hi, 7 * Ns); PJ_H(T_mid, Ns); PJ_H(mw_post, Ns - 1);
+#undef PJ_H
+    return h;
 }
 
 }  // namespace
@@ -743,6 +812,45 @@ int pyjac_hip_write_data(const char* filename, const double* arr, size_t count) 
     return PYJAC_OK;
 }
 
+int pyjac_hip_tables_hash(const pyjac_mech_tables* tables, unsigned long long* out) {
+    if (!tables || !out) return fail(PYJAC_ERR_ARG, "null argument");
+    *out = tables_fingerprint(tables);
+    return PYJAC_OK;
+}
+
+int pyjac_hip_load_module(pyjac_handle* h, const char* path) {
+    if (!h || !path) return fail(PYJAC_ERR_ARG, "null argument");
+    hipModule_t mod = nullptr;
+    if (hipModuleLoad(&mod, path) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(PYJAC_ERR_IO, "could not load code object %s", path);
+    }
+    auto reject = [&](int code, const char* why) {
+        (void)hipModuleUnload(mod);
+        return fail(code, "code object %s: %s", path, why);
+    };
+    hipDeviceptr_t sig_ptr = nullptr;
+    size_t sig_bytes = 0;
+    hipFunction_t fn = nullptr;
+    unsigned long long sig[4] = {0, 0, 0, 0};
+    if (hipModuleGetGlobal(&sig_ptr, &sig_bytes, mod, "pyjac_lane_sig") != hipSuccess ||
+        sig_bytes != sizeof(sig))
+        return reject(PYJAC_ERR_ARG, "not a specialised species-rate module");
+    if (hipMemcpyDtoH(sig, sig_ptr, sizeof(sig)) != hipSuccess)
+        return reject(PYJAC_ERR_HIP, "signature unreadable");
+    if (sig[1] != (unsigned long long)LANE_VERSION)
+        return reject(PYJAC_ERR_ARG, "generated for another module version");
+    if (sig[0] != h->tables_hash || sig[2] != (unsigned long long)h->m.Ns ||
+        sig[3] != (unsigned long long)h->m.Nr)
+        return reject(PYJAC_ERR_ARG, "generated from other mechanism tables");
+    if (hipModuleGetFunction(&fn, mod, "pyjac_lane_rates") != hipSuccess)
+        return reject(PYJAC_ERR_ARG, "no pyjac_lane_rates kernel");
+    if (h->lane_mod) (void)hipModuleUnload(h->lane_mod);
+    h->lane_mod = mod;
+    h->lane_fn = fn;
+    return PYJAC_OK;
+}
+
 int pyjac_hip_create(const pyjac_mech_tables* mt, int conp, char order, int jac_format,
                      int jac_type, pyjac_handle** out) {
     if (!mt || !out) return fail(PYJAC_ERR_ARG, "null argument");
@@ -762,6 +870,7 @@ int pyjac_hip_create(const pyjac_mech_tables* mt, int conp, char order, int jac_
         return fail(PYJAC_ERR_UNSUPPORTED, "Chebyshev degree above 16");
 
     pyjac_handle* h = new pyjac_handle();
+    h->tables_hash = tables_fingerprint(mt);
     if (mt->abi_version >= 2) {
         // names are optional; a table without them gives NULL names
         if (mt->species_names)
@@ -1221,6 +1330,7 @@ int pyjac_hip_destroy(pyjac_handle* h) {
     (void)hipFree(h->d_out2);
     (void)hipFree(h->d_out3);
     (void)hipFree(h->d_counters);
+    if (h->lane_mod) (void)hipModuleUnload(h->lane_mod);
     delete h;
     return PYJAC_OK;
 }
@@ -1599,6 +1709,7 @@ int pyjac_hip_autotune(pyjac_handle* h, int kernel, size_t n, const double* d_pa
         for (int w : {2, 4, 8}) cand.push_back(Cand{20 + w, 0});
     if (wave) cand.push_back(Cand{2, 0});
     if (!(kernel == 1 && h->bt.approx_sums)) cand.push_back(Cand{1, 0});
+    if (kernel == 0 && h->lane_fn) cand.push_back(Cand{30, 0});
     const int saved_variant = h->variant[kernel], saved_wps = h->wps[kernel];
     const int saved_wps_g = h->wps_g, saved_occ = h->occ[kernel];
     const long saved_chunk = h->fchunk_pref;
@@ -1658,12 +1769,15 @@ int pyjac_hip_autotune(pyjac_handle* h, int kernel, size_t n, const double* d_pa
     }
     h->occ[kernel] = best.occ;
     if (kernel == 1) h->fchunk_pref = best_chunk;
-    if (best.v > 20) {
+    if (best.v == 30) {
+        h->variant[kernel] = 0;  // the loaded module is the automatic choice
+    } else if (best.v > 20) {
         // automatic choice only when the dense accumulator does not fit
         h->variant[kernel] = h->block_ok_jac ? best.v : 0;
         h->wps_g = best.v - 20;
     } else if (best.v > 10) {
-        h->variant[kernel] = 0;
+        // with a module loaded the automatic choice would be the module
+        h->variant[kernel] = (kernel == 0 && h->lane_fn) ? best.v : 0;
         h->wps[kernel] = best.v - 10;
     } else {
         h->variant[kernel] = best.v;
@@ -1711,7 +1825,8 @@ int pyjac_hip_set_variant(pyjac_handle* h, int kernel, int variant) {
         const bool lane_ok = !(kernel == 1 && h->bt.approx_sums);
         bool ok = variant == 0 || (variant == 1 && lane_ok) || (variant == 2 && wave) ||
                   (variant > 10 && variant < 20 && wok && block) ||
-                  (kernel == 1 && variant > 20 && variant < 30 && wok && h->glob_ok_jac);
+                  (kernel == 1 && variant > 20 && variant < 30 && wok && h->glob_ok_jac) ||
+                  (kernel == 0 && variant == 30 && h->lane_fn);
         if (!ok)
             return fail(PYJAC_ERR_UNSUPPORTED, "variant %d cannot run this mechanism's %s", variant,
                         kernel == 0 ? "species rates" : "Jacobian");

@@ -31,6 +31,18 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__
 _lib = None
 
 
+def _share_hip_runtime():
+    """Load torch's HIP runtime first when torch is installed.  torch links
+    its bundled ``libamdhip64`` under a name that does not match a copy from
+    /opt/rocm already in the process, so loading this library first would put
+    two HIP runtimes in one process, and the second finds no device.  Loaded
+    after torch, the library binds to torch's copy."""
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+
+
 def load_library(path=None):
     """Load ``libpyjac_hip.so`` (raises if it has not been built)."""
     global _lib
@@ -41,6 +53,7 @@ def load_library(path=None):
         raise HipRuntimeError(-1, 'HIP library not built: {} (run '
                                   '__graft_entry__.build() or '
                                   'pyjac.libgen.build_library())'.format(path))
+    _share_hip_runtime()
     L = ctypes.CDLL(path)
     P, I, S = ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t
     L.pyjac_hip_last_error.restype = ctypes.c_char_p
@@ -81,6 +94,8 @@ def load_library(path=None):
     L.pyjac_hip_set_tuning.argtypes = [P, I, I, ctypes.c_long]
     L.pyjac_hip_get_tuning.argtypes = [P, I, ctypes.POINTER(I), ctypes.POINTER(ctypes.c_long),
                                        ctypes.POINTER(I)]
+    L.pyjac_hip_tables_hash.argtypes = [P, ctypes.POINTER(ctypes.c_ulonglong)]
+    L.pyjac_hip_load_module.argtypes = [P, ctypes.c_char_p]
     L.pyjac_hip_read_initial_conditions.argtypes = [ctypes.c_char_p, S, I, P, P, ctypes.c_char]
     L.pyjac_hip_write_data.argtypes = [ctypes.c_char_p, P, S]
     if path == LIB_PATH or _lib is None:
@@ -146,6 +161,10 @@ class _KernelBase(object):
                                         ctypes.byref(h)))
         self._h = h
         self.problem_size = 0
+        self.module = None
+        module = kwargs.pop('module', None)
+        if module is not None:
+            self.load_module(module)
         if self._variant_slot <= 1:
             self._apply_tuned()
         if not kwargs.pop('testing_only_skip_compilation', False):
@@ -156,6 +175,23 @@ class _KernelBase(object):
         if h is not None and h.value and self.lib is not None:
             self.lib.pyjac_hip_destroy(h)
             self._h = None
+
+    def load_module(self, module='auto'):
+        """Attach the mechanism-specialised species-rate module
+        (``pyjac.libgen.lanegen``): a code object path, or 'auto' for the
+        cached module of this mechanism's tables (built by
+        ``__graft_entry__.build()`` or ``pyjac --lang hip``; nothing is
+        compiled here).  The species rates, and the finite-difference
+        Jacobian's perturbed states, then run on it.  Returns the path, or
+        None when 'auto' finds no module."""
+        if module == 'auto':
+            from ..libgen import lanegen
+            module = lanegen.ensure_module(self.mech, build_missing=False)
+            if module is None:
+                return None
+        check(self.lib.pyjac_hip_load_module(self._h, os.fsencode(module)))
+        self.module = module
+        return module
 
     def species_names(self):
         """Species names in the kernel's order (bath gas last), from the

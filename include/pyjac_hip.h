@@ -356,6 +356,19 @@ int pyjac_hip_batch_set(pyjac_batch *b, int member, size_t n_states, const doubl
                         const double *d_phi, double *d_out);
 int pyjac_hip_batch_run(pyjac_batch *b, int kernel, int use_graph, void *stream);
 int pyjac_hip_batch_destroy(pyjac_batch *b);
+/* Jacobian batches in one launch (no counterpart in the reference; its
+   queue-based driver, pyjac/core/driver_kernels.py:383, is unimplemented):
+   with on = 1 every analytical-Jacobian member, whether its mechanism takes
+   the dense LDS accumulator or the device-memory gather, is evaluated by one
+   kernel launch (finite-difference members still get their own launches).
+   The gather members' workgroups are dispatched first and take their
+   contribution buffers from a pool with more slots than workgroups can be
+   resident; F-order members are written in place (no chunk transposes).  Up
+   to 8 members; the pool is allocated by the first run after a change.
+   slot_failures reports claims that found no free slot (always 0 unless the
+   pool is too small; such a state's output is not written). */
+int pyjac_hip_batch_set_queue(pyjac_batch *b, int on);
+int pyjac_hip_batch_slot_failures(pyjac_batch *b, int *out);
 
 /*
  * Autotuner (no counterpart in the reference): times every kernel family that

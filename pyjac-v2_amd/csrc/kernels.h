@@ -168,4 +168,22 @@ struct RaggedLaunch {
 hipError_t launch_ragged(int kernel, int wps, int count, const RaggedLaunch* members,
                          size_t lds, hipStream_t stream);
 
+// ---- ragged Jacobian batch in one launch, members of both Jacobian variants
+struct QueueLaunch {
+    const BlockArgs* args;  // the member's dense-kernel or gather-kernel BlockArgs
+    int conp, NN, gather;
+    char order;
+    long n, nout;           // states, stored values per state
+    const double *param, *phi;
+    double* out;
+};
+// busy: 8 * nslots + 1 zeroed ints (a count of failed slot claims, then the
+// slot flags, left zeroed by every launch); slots: 8 * nslots buffers of
+// `slot` doubles for the gather members; nslots per XCD, above the
+// workgroups resident on one XCD at once; wps 2, 4 or 8
+hipError_t launch_queue(int wps, int count, const QueueLaunch* members, size_t lds, int* busy,
+                        int nslots, double* slots, long slot, hipStream_t stream);
+// workgroups of the queue kernel that fit on one CU (0 on error)
+int queue_resident(int wps, size_t lds);
+
 }  // namespace pjk_host

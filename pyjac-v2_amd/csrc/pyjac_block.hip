@@ -1360,7 +1360,7 @@ __device__ __forceinline__ void jacobian_global_state(const BlockArgs* __restric
                                                       const double* __restrict__ phi,
                                                       double* __restrict__ cb,
                                                       double* __restrict__ out, Layout L,
-                                                      double* smem) {
+                                                      double* smem, long ostride = 1) {
     constexpr int NT = 64 * WPS;
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -1480,7 +1480,7 @@ __device__ __forceinline__ void jacobian_global_state(const BlockArgs* __restric
                 } else {
                     v = col == 0 ? col0[k] : col1[k];
                 }
-                out[e] = v;
+                out[e * ostride] = v;
             }
         }
     }
@@ -1502,8 +1502,8 @@ __device__ __forceinline__ void jacobian_global_state(const BlockArgs* __restric
         double j0 = -1.0 * (cs0[j] + su0) * inv_vh;
         j0 = j0 + -1.0 * Tdot * (c.heat(j) + -1.0 * h_ns) * inv_vh;
         const double j1 = T * RU * (cs1[j] + su1) / fixed + extra * j0 * Tinv;
-        out[amap[j + 2]] = j0;
-        out[amap[NN + j + 2]] = j1;
+        out[amap[j + 2] * ostride] = j0;
+        out[amap[NN + j + 2] * ostride] = j1;
     }
     if (wave == WPS - 1) {
         // columns 0 and 1 of rows 0 and 1 (create_jacobian.py:1595-2114)
@@ -1540,7 +1540,7 @@ __device__ __forceinline__ void jacobian_global_state(const BlockArgs* __restric
         }
         if (lane == 0) {
             const double j00 = (Tdot * dTsum + rate_sum) / heat_tot;
-            out[amap[0]] = j00;
+            out[amap[0] * ostride] = j00;
             double j10;
             if (conp) {
                 j10 = st.V * Tinv * (j00 + -1.0 * Tinv * Tdot);
@@ -1549,7 +1549,7 @@ __device__ __forceinline__ void jacobian_global_state(const BlockArgs* __restric
                 j10 = st.P * (j00 - Tdot * Tinv) * Tinv;
                 j10 = j10 + RU * sumT;
             }
-            out[amap[NN]] = j10;
+            out[amap[NN] * ostride] = j10;
             double j01;
             if (conp) {
                 j01 = (Tdot * dTsumE + -1.0 * specsum) * (1.0 / (heat_tot * st.V));
@@ -1558,11 +1558,11 @@ __device__ __forceinline__ void jacobian_global_state(const BlockArgs* __restric
                 j01 = -(Tdot * h_ns / (RU * T)) * spec_inv;
                 j01 = j01 - (sumE * Vinv) * spec_inv;
             }
-            out[amap[1]] = j01;
+            out[amap[1] * ostride] = j01;
             const double var = conp ? st.V : st.P;
             const double par = conp ? st.P : st.V;
             const double j11 = (var * j01 + Tdot) / T;
-            out[amap[NN + 1]] = j11 + RU * T * sumEE / par;
+            out[amap[NN + 1] * ostride] = j11 + RU * T * sumEE / par;
         }
     }
 }
@@ -1759,6 +1759,86 @@ jacobian_ragged_kernel(RaggedArgs R) {
     const long s = state_of_block(blockIdx.x - M.block0, M.n);
     if (s >= M.n) return;
     jacobian_state<WPS>(M.A, M.conp, M.n, s, M.param, M.phi, M.out, M.L, M.out_C, smem);
+}
+
+// ------------------------------------------ the ragged Jacobian in one launch
+// Members of either Jacobian variant -- dense LDS accumulator or the
+// contributions gathered in device memory -- share one launch: member i owns
+// workgroups [b0_i, b0_i + 8 ceil(n_i / 8)), the gather members first (the
+// most expensive states are dispatched first), and inside its range the
+// XCD-aware state order of the single-mechanism launch applies.  A gather
+// state needs a contribution buffer for as long as its workgroup runs; the
+// buffers form a pool with a busy flag per slot, split into one part per XCD
+// (read from the XCC_ID hardware register), each part holding twice the
+// workgroups that can be resident on that XCD: a workgroup claims a free
+// slot of its XCD's part with a compare-and-swap and frees it when it is
+// done, so a free slot always exists and the search ends.  A slot is only
+// ever reused within one XCD, whose L2 then holds its only copy, so handing
+// it on needs no write-back of the L2.  (A persistent loop over work items
+// around the per-state code spills 185 VGPRs; one state per workgroup
+// spills about as few as the single-mechanism kernels.)
+struct QueueMember {
+    const BlockArgs* A;
+    int conp, out_C, gather;
+    long n, b0;                 // states, first workgroup
+    const double *param, *phi;
+    double* out;
+    Layout L;
+    long nout;                  // gather: stored values per state
+};
+struct QueueArgs {
+    int count;
+    QueueMember m[pjk_host::RAGGED_MAX];
+    int* busy;                  // a count of failed claims, then 8 * nslots slot flags
+    int nslots;                 // slots per XCD
+    double* slots;              // gather members: nslots contribution buffers
+    long slot;                  // doubles per slot
+};
+
+constexpr long SLOT_PROBES = 1L << 24;  // claim attempts before giving up
+
+template <int WPS>
+__global__ void __launch_bounds__(64 * WPS) __attribute__((amdgpu_waves_per_eu(PJB_JAC_WAVES)))
+jacobian_queue_kernel(QueueArgs Q) {
+    extern __shared__ double smem[];
+    __shared__ int slot_sh;
+    int mi = 0;
+    for (int k = 1; k < Q.count; ++k)
+        if (Q.m[k].b0 <= (long)blockIdx.x) mi = k;
+    const QueueMember& M = Q.m[mi];
+    const long s = state_of_block(blockIdx.x - M.b0, M.n);
+    if (s >= M.n) return;
+    if (!M.gather) {
+        jacobian_state<WPS>(M.A, M.conp, M.n, s, M.param, M.phi, M.out, M.L, M.out_C, smem);
+        return;
+    }
+    if (threadIdx.x == 0) {
+        // hwreg(HW_REG_XCC_ID): the XCD this workgroup runs on, 0-7
+        const int xcc = __builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20) & 7;
+        int* flags = Q.busy + 1 + (size_t)xcc * Q.nslots;
+        int i = (int)((blockIdx.x >> 3) % (unsigned)Q.nslots), got = -1;
+        for (long t = 0; t < SLOT_PROBES; ++t) {
+            if (atomicCAS(flags + i, 0, 1) == 0) {
+                got = xcc * Q.nslots + i;
+                break;
+            }
+            i = i + 1 == Q.nslots ? 0 : i + 1;
+        }
+        if (got < 0) atomicAdd(Q.busy, 1);
+        slot_sh = got;
+    }
+    __syncthreads();
+    const int slot = __builtin_amdgcn_readfirstlane(slot_sh);
+    if (slot < 0) return;
+    double* cb = Q.slots + (size_t)slot * Q.slot;
+    double* out = M.out_C ? M.out + s * M.nout : M.out + s;
+    jacobian_global_state<WPS>(M.A, M.conp, s, M.param, M.phi, cb, out, M.L, smem,
+                               M.out_C ? 1 : M.n);
+    // every store to the buffer done (in this XCD's L2) before the slot is
+    // handed on; the next owner reads only what it writes itself
+    wait_memory();
+    __syncthreads();
+    if (threadIdx.x == 0) atomicExch(Q.busy + 1 + slot, 0);
 }
 
 }  // namespace pjb
@@ -2045,6 +2125,63 @@ hipError_t launch_jacobian_global(const DevMech& m, const BlockArgs* args, const
         if (e != hipSuccess) return e;
     }
     return hipSuccess;
+}
+
+hipError_t launch_queue(int wps, int count, const QueueLaunch* members, size_t lds, int* busy,
+                        int nslots, double* slots, long slot, hipStream_t stream) {
+    if (count < 1 || count > RAGGED_MAX || nslots < 1) return hipErrorInvalidValue;
+    pjb::QueueArgs Q{};
+    Q.count = count;
+    Q.busy = busy;
+    Q.nslots = nslots;
+    Q.slots = slots;
+    Q.slot = slot;
+    long blocks = 0;
+    for (int i = 0; i < count; ++i) {
+        const QueueLaunch& q = members[i];
+        pjb::Layout L = q.order == 'F' ? pjb::Layout{1, (size_t)q.n}
+                                       : pjb::Layout{(size_t)q.NN, 1};
+        Q.m[i] = pjb::QueueMember{q.args, q.conp, q.order == 'C', q.gather, q.n, blocks, q.param,
+                                  q.phi, q.out, L, q.nout};
+        blocks += 8 * ((q.n + 7) / 8);
+    }
+    if (blocks == 0) return hipSuccess;
+#define PJB_Q(W)                                                                           \
+    case W: {                                                                              \
+        hipError_t e = set_lds(pjb::jacobian_queue_kernel<W>, lds);                        \
+        if (e != hipSuccess) return e;                                                     \
+        hipLaunchKernelGGL(pjb::jacobian_queue_kernel<W>, dim3(blocks), dim3(64 * W), lds, \
+                           stream, Q);                                                     \
+        break;                                                                             \
+    }
+    switch (wps) {
+        PJB_Q(2)
+        PJB_Q(4)
+        PJB_Q(8)
+        default: return hipErrorInvalidValue;
+    }
+#undef PJB_Q
+    return hipGetLastError();
+}
+
+int queue_resident(int wps, size_t lds) {
+    int nb = 0;
+    hipError_t e = hipSuccess;
+#define PJB_QO(W)                                                                         \
+    case W:                                                                               \
+        e = set_lds(pjb::jacobian_queue_kernel<W>, lds);                                  \
+        if (e == hipSuccess)                                                              \
+            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(                             \
+                &nb, pjb::jacobian_queue_kernel<W>, 64 * W, lds);                         \
+        break;
+    switch (wps) {
+        PJB_QO(2)
+        PJB_QO(4)
+        PJB_QO(8)
+        default: return 0;
+    }
+#undef PJB_QO
+    return e == hipSuccess ? nb : 0;
 }
 
 hipError_t launch_ragged(int kernel, int wps, int count, const RaggedLaunch* members,

@@ -1566,6 +1566,14 @@ struct pyjac_batch {
     hipGraph_t graph[2] = {nullptr, nullptr};
     hipGraphExec_t exec[2] = {nullptr, nullptr};
     bool dirty[2] = {true, true};
+    // Jacobian in one launch for members of both variants: the slot flags
+    // (zeroed, and left zeroed by every launch) and the gather members'
+    // contribution buffers
+    int queue = 0;
+    int* d_busy = nullptr;
+    int nslots = 0;
+    double* d_slots = nullptr;
+    size_t slot_doubles = 0;
 };
 
 namespace {
@@ -1578,8 +1586,104 @@ void drop_graph(pyjac_batch* b, int k) {
     b->dirty[k] = true;
 }
 
+// the members the queue kernel takes: exact or approximate analytical
+// Jacobians that the dense or the gather variant can run
+bool queue_member(const pyjac_handle* h) {
+    return h->jac_type != PYJAC_JAC_FINITE_DIFFERENCE && (use_block(h, 1) || use_glob(h));
+}
+
+struct QueuePlan {
+    std::vector<pjk_host::QueueLaunch> q;
+    int wps = 2, nslots = 0;
+    size_t lds = 0, slot = 0;
+};
+
+// the one-launch plan of batch b: members (the gather members, then the
+// others by falling reaction count: the most expensive states are dispatched
+// first), waves per state, LDS, contribution slots
+int queue_plan(pyjac_batch* b, QueuePlan& P) {
+    std::vector<int> idx;
+    for (size_t i = 0; i < b->h.size(); ++i)
+        if (b->mem[i].n && queue_member(b->h[i])) idx.push_back((int)i);
+    if (idx.empty()) return PYJAC_OK;
+    if (idx.size() > (size_t)pjk_host::RAGGED_MAX)
+        return fail(PYJAC_ERR_UNSUPPORTED, "one launch takes up to %d members",
+                    pjk_host::RAGGED_MAX);
+    std::stable_sort(idx.begin(), idx.end(), [&](int a, int c) {
+        const bool ga = use_glob(b->h[a]), gc = use_glob(b->h[c]);
+        if (ga != gc) return ga;
+        return b->h[a]->m.Nr > b->h[c]->m.Nr;
+    });
+    for (int i : idx) {
+        const pyjac_handle* h = b->h[i];
+        const bool g = use_glob(h);
+        P.wps = std::max(P.wps, g ? glob_wps(h) : block_wps(h, 1));
+        P.lds = std::max(P.lds, (size_t)(g ? h->lds_g.total : h->lds[1].total) * sizeof(double));
+        if (g) P.slot = std::max(P.slot, (size_t)h->bt.n_contrib);
+    }
+    if (P.wps > 4) P.wps = 8;
+    else if (P.wps > 2) P.wps = 4;
+    const int per_cu = pjk_host::queue_resident(P.wps, P.lds);
+    if (per_cu < 1) return fail(PYJAC_ERR_UNSUPPORTED, "batch kernel does not fit on a CU");
+    // per XCD, twice the workgroups that can be resident on it: a claim
+    // always finds a slot
+    P.nslots = P.slot ? 2 * per_cu * ((b->h[idx[0]]->n_cu + 7) / 8) : 1;
+    for (int i : idx) {
+        const pyjac_handle* h = b->h[i];
+        const auto& m = b->mem[i];
+        const bool g = use_glob(h);
+        P.q.push_back(pjk_host::QueueLaunch{g ? h->d_args_g : h->d_args[1], h->conp, h->m.NN,
+                                            g ? 1 : 0, h->order, (long)m.n, h->m.jac_nnz,
+                                            m.param, m.phi, m.out});
+    }
+    return PYJAC_OK;
+}
+
+// the one-launch buffers, before any capture
+int prepare_queue(pyjac_batch* b) {
+    QueuePlan P;
+    int rc = queue_plan(b, P);
+    if (rc || P.q.empty()) return rc;
+    if (P.nslots > b->nslots) {
+        (void)hipFree(b->d_busy);
+        b->d_busy = nullptr;
+        b->nslots = 0;
+        HIP_TRY(hipMalloc(&b->d_busy, (size_t)(8 * P.nslots + 1) * sizeof(int)));
+        HIP_TRY(hipMemset(b->d_busy, 0, (size_t)(8 * P.nslots + 1) * sizeof(int)));
+        b->nslots = P.nslots;
+    }
+    const size_t need = P.slot * (size_t)P.nslots * 8;
+    if (need > b->slot_doubles) {
+        (void)hipFree(b->d_slots);
+        b->d_slots = nullptr;
+        b->slot_doubles = 0;
+        HIP_TRY(hipMalloc(&b->d_slots, std::max<size_t>(need, 1) * sizeof(double)));
+        b->slot_doubles = need;
+    }
+    return PYJAC_OK;
+}
+
 // every launch of one batch evaluation, on stream s
 int batch_launches(pyjac_batch* b, int kernel, hipStream_t s) {
+    if (kernel == 1 && b->queue) {
+        QueuePlan P;
+        int rc = queue_plan(b, P);
+        if (rc) return rc;
+        if (!P.q.empty()) {
+            if (!b->d_busy || P.nslots > b->nslots ||
+                P.slot * (size_t)P.nslots * 8 > b->slot_doubles)
+                return fail(PYJAC_ERR_STATE, "batch buffers not prepared");
+            HIP_TRY(pjk_host::launch_queue(P.wps, (int)P.q.size(), P.q.data(), P.lds, b->d_busy,
+                                           P.nslots, b->d_slots, (long)P.slot, s));
+        }
+        // members the queue kernel does not take (finite differences)
+        for (size_t i = 0; i < b->h.size(); ++i) {
+            const auto& m = b->mem[i];
+            if (!m.n || queue_member(b->h[i])) continue;
+            if ((rc = run_jac(b->h[i], (long)m.n, m.param, m.phi, m.out, s))) return rc;
+        }
+        return PYJAC_OK;
+    }
     std::vector<pjk_host::RaggedLaunch> rag;
     size_t lds = 0;
     int wps = 1;
@@ -1639,9 +1743,30 @@ int pyjac_hip_batch_set(pyjac_batch* b, int i, size_t n_states, const double* d_
     return PYJAC_OK;
 }
 
+int pyjac_hip_batch_slot_failures(pyjac_batch* b, int* out) {
+    if (!b || !out) return fail(PYJAC_ERR_ARG, "null argument");
+    *out = 0;
+    if (!b->d_busy) return PYJAC_OK;
+    HIP_TRY(hipMemcpy(out, b->d_busy, sizeof(int), hipMemcpyDeviceToHost));
+    return PYJAC_OK;
+}
+
+int pyjac_hip_batch_set_queue(pyjac_batch* b, int on) {
+    if (!b) return fail(PYJAC_ERR_ARG, "null batch");
+    if (b->queue != (on ? 1 : 0)) {
+        b->queue = on ? 1 : 0;
+        drop_graph(b, 1);
+    }
+    return PYJAC_OK;
+}
+
 int pyjac_hip_batch_run(pyjac_batch* b, int kernel, int use_graph, void* stream) {
     if (!b || (kernel != 0 && kernel != 1)) return fail(PYJAC_ERR_ARG, "bad batch call");
     hipStream_t s = (hipStream_t)stream;
+    if (kernel == 1 && b->queue) {
+        int rc = prepare_queue(b);
+        if (rc) return rc;
+    }
     if (!use_graph) return batch_launches(b, kernel, s);
     if (b->dirty[kernel]) {
         // allocations and kernel attributes first: nothing may allocate
@@ -1676,6 +1801,8 @@ int pyjac_hip_batch_destroy(pyjac_batch* b) {
     drop_graph(b, 0);
     drop_graph(b, 1);
     if (b->cap) (void)hipStreamDestroy(b->cap);
+    (void)hipFree(b->d_busy);
+    (void)hipFree(b->d_slots);
     delete b;
     return PYJAC_OK;
 }

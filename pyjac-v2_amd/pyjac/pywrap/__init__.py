@@ -88,6 +88,8 @@ def load_library(path=None):
     L.pyjac_hip_batch_set.argtypes = [P, I, S, P, P, P]
     L.pyjac_hip_batch_run.argtypes = [P, I, I, P]
     L.pyjac_hip_batch_destroy.argtypes = [P]
+    L.pyjac_hip_batch_set_queue.argtypes = [P, I]
+    L.pyjac_hip_batch_slot_failures.argtypes = [P, ctypes.POINTER(I)]
     L.pyjac_hip_autotune.argtypes = [P, I, S, P, P, P, P, ctypes.POINTER(I),
                                      ctypes.POINTER(ctypes.c_float)]
     L.pyjac_hip_set_fd_options.argtypes = [P, I, I, ctypes.c_double, ctypes.c_double]
@@ -403,6 +405,18 @@ class Batch(object):
     def run(self, graph=True):
         check(self.lib.pyjac_hip_batch_run(self._b, self.kernels[0]._variant_slot, int(graph),
                                            _current_stream_ptr()))
+
+    def set_queue(self, on=True):
+        """Jacobian batches: evaluate every analytical-Jacobian member, of
+        either Jacobian variant, in one launch (``pyjac_hip_batch_set_queue``)."""
+        check(self.lib.pyjac_hip_batch_set_queue(self._b, int(bool(on))))
+
+    def slot_failures(self):
+        """Contribution-slot claims of the one-launch kernel that found no
+        free slot (0 unless the pool is too small)."""
+        v = ctypes.c_int()
+        check(self.lib.pyjac_hip_batch_slot_failures(self._b, ctypes.byref(v)))
+        return int(v.value)
 
     def __del__(self):
         b = getattr(self, '_b', None)

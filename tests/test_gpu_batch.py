@@ -36,13 +36,33 @@ def _setup(mechs, jac):
     return kernels, arrays
 
 
-def _same(o, r, jac):
-    """Rates: bit-identical.  Jacobians: the variant that accumulates in
-    device memory (the large mechanisms) adds with L2 atomics, whose order
-    varies from run to run, so each state agrees to 1e-13 (relative
-    Frobenius) instead."""
+# states whose entries come this close to the largest double overflow or not
+# depending on the order of their sums
+NEAR_OVERFLOW = 1e300
+
+
+def _same(o, r, jac, n=None, order='C'):
+    """Rates: bit-identical.  Jacobians: the dense kernel adds into LDS with
+    atomics whose order depends on the waves per state (and varies from run
+    to run with more than one), so each entry agrees to 1e-13 of the
+    member's largest entry instead.  test.inp's random states include some
+    whose entries reach 1e304 or overflow (the checker's too); whether such a
+    state overflows depends on the order of its sums, so states near
+    overflow in either result need only be near overflow in both."""
     if not jac or torch.equal(o, r):
         return torch.equal(o, r)
+    if n is not None:
+        rs = r.view(n, -1) if order == 'C' else r.view(-1, n).t()
+        os_ = o.view(n, -1) if order == 'C' else o.view(-1, n).t()
+
+        def near(a):
+            big = torch.where(torch.isfinite(a), a.abs(), torch.full_like(a, float('inf')))
+            return big.max(dim=1).values > NEAR_OVERFLOW
+        nr, no = near(rs), near(os_)
+        if not bool((nr == no).all()):
+            # one result near overflow, the other not: a real difference
+            return False
+        r, o = rs[~nr], os_[~nr]
     diff = o != r
     if not bool(torch.isfinite(r[diff]).all()):
         return False  # a non-finite entry that differs
@@ -60,11 +80,13 @@ def _eager(kernels, arrays):
     return res
 
 
-@pytest.mark.parametrize('jac', [False, True])
-def test_batch_equals_single_calls(mechs, jac):
+@pytest.mark.parametrize('jac,one_launch', [(False, False), (True, False), (True, True)])
+def test_batch_equals_single_calls(mechs, jac, one_launch):
     kernels, arrays = _setup(mechs, jac)
     ref = _eager(kernels, arrays)
     b = Batch(kernels)
+    if one_launch:
+        b.set_queue(True)
     for i, (p, f, o) in enumerate(arrays):
         b.set(i, p, f, o)
     for graph in (False, True):
@@ -72,15 +94,17 @@ def test_batch_equals_single_calls(mechs, jac):
             o.fill_(np.nan)
         b.run(graph=graph)
         torch.cuda.synchronize()
-        for (_, _, o), r in zip(arrays, ref):
-            assert _same(o, r, jac)
+        for (_, _, o), r, mem in zip(arrays, ref, MEMBERS):
+            assert _same(o, r, jac, mem[4], mem[2]), mem
 
 
-def test_graph_replays_follow_new_inputs(mechs):
+@pytest.mark.parametrize('one_launch', [False, True])
+def test_graph_replays_follow_new_inputs(mechs, one_launch):
     """Replays of the captured graph, with eager work in between, see the
     inputs as they are at replay time."""
     kernels, arrays = _setup(mechs, True)
     b = Batch(kernels)
+    b.set_queue(one_launch)
     for i, (p, f, o) in enumerate(arrays):
         b.set(i, p, f, o)
     for rep in range(3):
@@ -94,8 +118,8 @@ def test_graph_replays_follow_new_inputs(mechs):
         ref = _eager(kernels, arrays)
         b.run(graph=True)
         torch.cuda.synchronize()
-        for (_, _, o), r in zip(arrays, ref):
-            assert _same(o, r, True), rep
+        for (_, _, o), r, mem in zip(arrays, ref, MEMBERS):
+            assert _same(o, r, True, mem[4], mem[2]), (rep, mem)
 
 
 def test_autotune_keeps_results_right(mechs, tmp_path, monkeypatch):

@@ -64,7 +64,8 @@ MEMBERS = [('h2o2', 200000), ('gri30', 600000), ('s150', 200000)]
 
 
 @pytest.mark.timeout(900)
-def test_ragged_batch_at_config_sizes(mechs):
+@pytest.mark.parametrize('one_launch', [False, True])
+def test_ragged_batch_at_config_sizes(mechs, one_launch):
     kernels, arrays, host = [], [], []
     for i, (name, n) in enumerate(MEMBERS):
         mech = mechs(name)
@@ -77,14 +78,21 @@ def test_ragged_batch_at_config_sizes(mechs):
         arrays.append((torch.from_numpy(param).cuda(), torch.from_numpy(phi).cuda(), out))
         host.append((mech, param, phi))
     b = Batch(kernels)
+    b.set_queue(one_launch)
     for i, (p, f, o) in enumerate(arrays):
         b.set(i, p, f, o)
     b.run(graph=True)   # capture and replay
     torch.cuda.synchronize()
     for k, (_, _, o) in zip(kernels, arrays):
         o.fill_(np.nan)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
     b.run(graph=True)   # replay
+    e1.record()
     torch.cuda.synchronize()
+    print('c2: one launch' if one_launch else 'c2: per-variant launches',
+          'replay %.1f ms' % e0.elapsed_time(e1), flush=True)
+    assert b.slot_failures() == 0
     rng = np.random.default_rng(99)
     for (name, n), k, (mech, param, phi), (_, _, out) in zip(MEMBERS, kernels, host, arrays):
         print('c2: checking', name, flush=True)

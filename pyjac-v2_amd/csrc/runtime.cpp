@@ -123,6 +123,17 @@ struct pyjac_handle {
     unsigned long long tables_hash = 0;
     hipModule_t lane_mod = nullptr;
     hipFunction_t lane_fn = nullptr;
+    // mechanism-specialised analytical Jacobian module (variant 31): kernel
+    // A (rates and per-reaction values into scratch), kernel B per output
+    // layout, the scratch values per state and the scratch stride (states
+    // per chunk, a literal in the module)
+    std::vector<hipModule_t> ljac_mods;
+    hipFunction_t ljac_a = nullptr, ljac_b[4] = {nullptr, nullptr, nullptr, nullptr};
+    long ljac_nscr = 0, ljac_stride = 0;
+    double* d_ljac_scr = nullptr;
+    size_t ljac_scr_doubles = 0;
+    // the stored sparsity pattern, row * NN + col, whatever the format
+    std::vector<int> pattern;
 };
 
 namespace {
@@ -194,8 +205,26 @@ bool use_module(const pyjac_handle* h) {
     return h->lane_fn && (h->variant[0] == 0 || h->variant[0] == 30);
 }
 
+// the analytical Jacobian by the loaded specialised module: asked for, or
+// automatic once a module is loaded (exact Jacobians; the approximate pattern
+// and the finite-difference Jacobian keep the table-driven kernels)
+int ljac_layout(const pyjac_handle* h) {
+    return (h->jac_format == PYJAC_JAC_FULL ? 2 : 0) + (h->order == 'F' ? 1 : 0);
+}
+
+// the module's kernels for this handle are loaded (kernel A and the store
+// kernel of the handle's layout), for the exact Jacobian
+bool jac_module_ready(const pyjac_handle* h) {
+    return h->ljac_a && h->ljac_b[ljac_layout(h)] && h->jac_type == PYJAC_JAC_EXACT;
+}
+
+bool use_jac_module(const pyjac_handle* h) {
+    return jac_module_ready(h) && (h->variant[1] == 0 || h->variant[1] == 31);
+}
+
 bool use_block(const pyjac_handle* h, int kernel) {
     if (kernel == 0 && use_module(h)) return false;
+    if (kernel == 1 && use_jac_module(h)) return false;
     int v = h->variant[kernel];
     bool ok = kernel == 0 ? h->block_ok_rates : h->block_ok_jac;
     return ok && (v == 0 || (v > 10 && v < 20));
@@ -388,6 +417,7 @@ int prepare_lane(pyjac_handle* h, int kernel, long n) {
 // the specialised module's kernel: one state per lane, 64 per workgroup;
 // entry i of state s at [s * sN + i * sI]
 constexpr int LANE_VERSION = 1;
+constexpr int LJAC_VERSION = 1;
 constexpr int LANE_BLOCK = 64;
 
 int launch_module(pyjac_handle* h, long n, const double* param, const double* phi, double* dphi,
@@ -474,10 +504,45 @@ int run_fd_jac(pyjac_handle* h, long n, const double* param, const double* phi, 
     return PYJAC_OK;
 }
 
+// the specialised Jacobian module: per chunk of at most `ljac_stride`
+// states, kernel A writes the rates' per-reaction and per-species values to
+// the scratch array ([value][state in chunk]) and kernel B stores every entry
+// in the handle's layout; one state per lane, 64 per workgroup
+int launch_jac_module(pyjac_handle* h, long n, const double* param, const double* phi,
+                      double* jac, hipStream_t s) {
+    const long stride = h->ljac_stride;
+    const size_t need = (size_t)(h->ljac_nscr - 1) * stride + (size_t)std::min(n, stride);
+    if (need > h->ljac_scr_doubles) {
+        if (h->d_ljac_scr) (void)hipFree(h->d_ljac_scr);
+        h->d_ljac_scr = nullptr;
+        h->ljac_scr_doubles = 0;
+        HIP_TRY(hipMalloc(&h->d_ljac_scr, need * sizeof(double)));
+        h->ljac_scr_doubles = need;
+    }
+    const bool F = h->order == 'F';
+    const int layout = ljac_layout(h);
+    long sN = F ? 1 : h->m.NN, sI = F ? n : 1;
+    long jN = F ? 1 : nvalues(h);
+    int conp = h->conp;
+    double* scr = h->d_ljac_scr;
+    for (long s0 = 0; s0 < n; s0 += stride) {
+        long cnt = std::min(stride, n - s0);
+        const unsigned grid = (unsigned)((cnt + LANE_BLOCK - 1) / LANE_BLOCK);
+        void* aa[] = {&cnt, &s0, (void*)&param, (void*)&phi, &scr, &sN, &sI, (void*)&stride, &conp};
+        HIP_TRY(hipModuleLaunchKernel(h->ljac_a, grid, 1, 1, LANE_BLOCK, 1, 1, 0, s, aa, nullptr));
+        void* ab[] = {&cnt, &s0, (void*)&param, (void*)&phi, &scr, &jac, &sN, &sI, (void*)&stride,
+                      &jN, &conp};
+        HIP_TRY(hipModuleLaunchKernel(h->ljac_b[layout], grid, 1, 1, LANE_BLOCK, 1, 1, 0, s, ab,
+                                      nullptr));
+    }
+    return PYJAC_OK;
+}
+
 int run_jac(pyjac_handle* h, long n, const double* param, const double* phi, double* jac,
             hipStream_t s) {
     if (n <= 0) return PYJAC_OK;
     if (h->jac_type == PYJAC_JAC_FINITE_DIFFERENCE) return run_fd_jac(h, n, param, phi, jac, s);
+    if (use_jac_module(h)) return launch_jac_module(h, n, param, phi, jac, s);
     if (use_block(h, 1)) {
         int rc = ensure_fchunk(h, n);
         if (rc) return rc;
@@ -833,9 +898,61 @@ int pyjac_hip_load_module(pyjac_handle* h, const char* path) {
     size_t sig_bytes = 0;
     hipFunction_t fn = nullptr;
     unsigned long long sig[4] = {0, 0, 0, 0};
+    if (hipModuleGetGlobal(&sig_ptr, &sig_bytes, mod, "pyjac_ljac_sig") == hipSuccess) {
+        // a specialised analytical Jacobian module
+        unsigned long long js[7] = {0, 0, 0, 0, 0, 0, 0};
+        if (sig_bytes != sizeof(js) || hipMemcpyDtoH(js, sig_ptr, sizeof(js)) != hipSuccess)
+            return reject(PYJAC_ERR_ARG, "unreadable Jacobian module signature");
+        if (js[1] != (unsigned long long)LJAC_VERSION)
+            return reject(PYJAC_ERR_ARG, "generated for another module version");
+        if (js[0] != h->tables_hash || js[2] != (unsigned long long)h->m.Ns ||
+            js[3] != (unsigned long long)h->m.Nr)
+            return reject(PYJAC_ERR_ARG, "generated from other mechanism tables");
+        if (h->jac_type == PYJAC_JAC_APPROXIMATE)
+            return reject(PYJAC_ERR_UNSUPPORTED, "the module stores the exact Jacobian's pattern");
+        // its entries must be exactly the handle's stored pattern
+        const long ne = (long)js[5];
+        hipDeviceptr_t ent_ptr = nullptr;
+        size_t ent_bytes = 0;
+        std::vector<int> ent(ne > 0 ? ne : 0);
+        if (ne != (long)h->pattern.size() || js[4] < 1 || js[6] < 64 || js[6] % 64 ||
+            hipModuleGetGlobal(&ent_ptr, &ent_bytes, mod, "pyjac_ljac_entries") != hipSuccess ||
+            ent_bytes != ent.size() * sizeof(int) ||
+            hipMemcpyDtoH(ent.data(), ent_ptr, ent_bytes) != hipSuccess)
+            return reject(PYJAC_ERR_ARG, "entry list does not match the tables' sparsity pattern");
+        std::vector<char> seen((size_t)h->m.NN * h->m.NN, 0);
+        for (int e : h->pattern) seen[e] = 1;
+        for (int e : ent) {
+            const int row = e >> 16, col = e & 0xffff;
+            if (row >= h->m.NN || col >= h->m.NN || seen[row * h->m.NN + col] != 1)
+                return reject(PYJAC_ERR_ARG, "entry list does not match the tables' sparsity pattern");
+            seen[row * h->m.NN + col] = 2;
+        }
+        // the module is split into parts (kernel A; the store kernel of each
+        // layout): attach every kernel this code object holds
+        if (h->ljac_nscr && (h->ljac_nscr != (long)js[4] || h->ljac_stride != (long)js[6]))
+            return reject(PYJAC_ERR_ARG, "part of a different Jacobian module");
+        hipFunction_t a = nullptr, b[4] = {nullptr, nullptr, nullptr, nullptr};
+        bool any = hipModuleGetFunction(&a, mod, "pyjac_ljac_a") == hipSuccess;
+        for (int L = 0; L < 4; ++L) {
+            char name[32];
+            std::snprintf(name, sizeof(name), "pyjac_ljac_b%d", L);
+            if (hipModuleGetFunction(&b[L], mod, name) == hipSuccess) any = true;
+            else b[L] = nullptr;
+        }
+        (void)hipGetLastError();
+        if (!any) return reject(PYJAC_ERR_ARG, "no Jacobian kernel in the code object");
+        h->ljac_mods.push_back(mod);
+        if (a) h->ljac_a = a;
+        for (int L = 0; L < 4; ++L)
+            if (b[L]) h->ljac_b[L] = b[L];
+        h->ljac_nscr = (long)js[4];
+        h->ljac_stride = (long)js[6];
+        return PYJAC_OK;
+    }
     if (hipModuleGetGlobal(&sig_ptr, &sig_bytes, mod, "pyjac_lane_sig") != hipSuccess ||
         sig_bytes != sizeof(sig))
-        return reject(PYJAC_ERR_ARG, "not a specialised species-rate module");
+        return reject(PYJAC_ERR_ARG, "not a specialised species-rate or Jacobian module");
     if (hipMemcpyDtoH(sig, sig_ptr, sizeof(sig)) != hipSuccess)
         return reject(PYJAC_ERR_HIP, "signature unreadable");
     if (sig[1] != (unsigned long long)LANE_VERSION)
@@ -889,6 +1006,10 @@ int pyjac_hip_create(const pyjac_mech_tables* mt, int conp, char order, int jac_
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, h->device) == hipSuccess && prop.multiProcessorCount > 0)
         h->n_cu = prop.multiProcessorCount;
+
+    for (int row = 0; row < NN; ++row)
+        for (int q = mt->crs_row_ptr[row]; q < mt->crs_row_ptr[row + 1]; ++q)
+            h->pattern.push_back(row * NN + mt->crs_col_ind[q]);
 
     // derived tables
     std::vector<int> thd_only, fall_order, jac_pos;
@@ -1331,6 +1452,8 @@ int pyjac_hip_destroy(pyjac_handle* h) {
     (void)hipFree(h->d_out3);
     (void)hipFree(h->d_counters);
     if (h->lane_mod) (void)hipModuleUnload(h->lane_mod);
+    for (hipModule_t m : h->ljac_mods) (void)hipModuleUnload(m);
+    (void)hipFree(h->d_ljac_scr);
     delete h;
     return PYJAC_OK;
 }
@@ -1837,6 +1960,7 @@ int pyjac_hip_autotune(pyjac_handle* h, int kernel, size_t n, const double* d_pa
     if (wave) cand.push_back(Cand{2, 0});
     if (!(kernel == 1 && h->bt.approx_sums)) cand.push_back(Cand{1, 0});
     if (kernel == 0 && h->lane_fn) cand.push_back(Cand{30, 0});
+    if (kernel == 1 && jac_module_ready(h)) cand.push_back(Cand{31, 0});
     const int saved_variant = h->variant[kernel], saved_wps = h->wps[kernel];
     const int saved_wps_g = h->wps_g, saved_occ = h->occ[kernel];
     const long saved_chunk = h->fchunk_pref;
@@ -1896,7 +2020,7 @@ int pyjac_hip_autotune(pyjac_handle* h, int kernel, size_t n, const double* d_pa
     }
     h->occ[kernel] = best.occ;
     if (kernel == 1) h->fchunk_pref = best_chunk;
-    if (best.v == 30) {
+    if (best.v == 30 || best.v == 31) {
         h->variant[kernel] = 0;  // the loaded module is the automatic choice
     } else if (best.v > 20) {
         // automatic choice only when the dense accumulator does not fit
@@ -1904,7 +2028,9 @@ int pyjac_hip_autotune(pyjac_handle* h, int kernel, size_t n, const double* d_pa
         h->wps_g = best.v - 20;
     } else if (best.v > 10) {
         // with a module loaded the automatic choice would be the module
-        h->variant[kernel] = (kernel == 0 && h->lane_fn) ? best.v : 0;
+        const bool mod = kernel == 0 ? h->lane_fn != nullptr
+                                     : jac_module_ready(h);
+        h->variant[kernel] = mod ? best.v : 0;
         h->wps[kernel] = best.v - 10;
     } else {
         h->variant[kernel] = best.v;
@@ -1953,7 +2079,8 @@ int pyjac_hip_set_variant(pyjac_handle* h, int kernel, int variant) {
         bool ok = variant == 0 || (variant == 1 && lane_ok) || (variant == 2 && wave) ||
                   (variant > 10 && variant < 20 && wok && block) ||
                   (kernel == 1 && variant > 20 && variant < 30 && wok && h->glob_ok_jac) ||
-                  (kernel == 0 && variant == 30 && h->lane_fn);
+                  (kernel == 0 && variant == 30 && h->lane_fn) ||
+                  (kernel == 1 && variant == 31 && jac_module_ready(h));
         if (!ok)
             return fail(PYJAC_ERR_UNSUPPORTED, "variant %d cannot run this mechanism's %s", variant,
                         kernel == 0 ? "species rates" : "Jacobian");

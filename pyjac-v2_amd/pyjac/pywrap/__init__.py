@@ -164,6 +164,7 @@ class _KernelBase(object):
         self._h = h
         self.problem_size = 0
         self.module = None
+        self.jac_module = None
         module = kwargs.pop('module', None)
         if module is not None:
             self.load_module(module)
@@ -179,21 +180,46 @@ class _KernelBase(object):
             self._h = None
 
     def load_module(self, module='auto'):
-        """Attach the mechanism-specialised species-rate module
-        (``pyjac.libgen.lanegen``): a code object path, or 'auto' for the
-        cached module of this mechanism's tables (built by
+        """Attach mechanism-specialised code objects: a species-rate module
+        (``pyjac.libgen.lanegen``) or an analytical Jacobian module
+        (``pyjac.libgen.lanejac``), by path, or 'auto' for the cached
+        modules of this mechanism's tables (built by
         ``__graft_entry__.build()`` or ``pyjac --lang hip``; nothing is
-        compiled here).  The species rates, and the finite-difference
-        Jacobian's perturbed states, then run on it.  Returns the path, or
-        None when 'auto' finds no module."""
+        compiled here): the species-rate module always, the Jacobian module
+        for a Jacobian kernel of the exact Jacobian.  The species rates (and
+        the finite-difference Jacobian's perturbed states) then run on the
+        first, the analytical Jacobian on the second.  Returns the species-
+        rate module's path (the Jacobian module's is ``jac_module``), or None
+        when 'auto' finds neither."""
         if module == 'auto':
-            from ..libgen import lanegen
-            module = lanegen.ensure_module(self.mech, build_missing=False)
-            if module is None:
-                return None
-        check(self.lib.pyjac_hip_load_module(self._h, os.fsencode(module)))
-        self.module = module
+            from ..libgen import lanegen, lanejac
+            rates = lanegen.ensure_module(self.mech, build_missing=False)
+            jac = None
+            if self._variant_slot == 1 and self.jac_type == JacobianType.exact:
+                jac = lanejac.ensure_module(self.mech, build_missing=False)
+            if rates:
+                check(self.lib.pyjac_hip_load_module(self._h, os.fsencode(rates)))
+            if jac:
+                self._load_jac_parts(jac)
+            self.module = rates
+            self.jac_module = jac
+            return rates if rates or not jac else jac
+        if os.path.basename(module).startswith('ljac_'):
+            self._load_jac_parts(module)
+            self.jac_module = module
+        else:
+            check(self.lib.pyjac_hip_load_module(self._h, os.fsencode(module)))
+            self.module = module
         return module
+
+    def _load_jac_parts(self, path):
+        """A Jacobian module is five code objects (lanejac.part_paths):
+        kernel A at ``path`` and the store kernel of each output layout
+        beside it; every one that exists is loaded."""
+        from ..libgen import lanejac
+        for part in lanejac.part_paths(path):
+            if part == path or os.path.exists(part):
+                check(self.lib.pyjac_hip_load_module(self._h, os.fsencode(part)))
 
     def species_names(self):
         """Species names in the kernel's order (bath gas last), from the

@@ -6,8 +6,9 @@ Workload (the configuration the headline metric is quoted on): GRI-Mech 3.0
 1,048,576 seeded states (T ~ U(1000, 2800) K, P log-uniform 1-50 atm,
 Dirichlet(0.3) mass fractions, V = 1 m^3), species rates plus the sparse
 (CCS, 1984 non-zeros) analytical Jacobian, all FP64.  A step is one pass of
-the Jacobian kernel and one pass of the species-rate kernel over the whole
-batch, inputs and outputs resident in HBM.
+the Jacobian and one pass of the species rates over the whole batch, inputs
+and outputs resident in HBM, by the mechanism-specialised modules when they
+are built (the table-driven block kernels are timed beside them).
 
 Prints one JSON line (see the keys below).  ``--gpus`` above 1 is rejected:
 this build runs on one GPU.
@@ -45,15 +46,18 @@ def parse():
     return ap.parse_args()
 
 
-TRAFFIC_FILE = os.path.join(ROOT, 'profiles', 'traffic_jacobian_gri30_conv_f_sparse.txt')
+# counter summaries of the two Jacobian paths (mechanism-specialised module;
+# table-driven block kernel)
+TRAFFIC_FILE_MODULE = os.path.join(ROOT, 'profiles', 'traffic_ljac_gri30_conv_f_sparse.txt')
+TRAFFIC_FILE_BLOCK = os.path.join(ROOT, 'profiles', 'traffic_jacobian_gri30_conv_f_sparse.txt')
 
 
-def measured_traffic(n):
-    """HBM bytes per Jacobian launch from the committed counter summary
+def measured_traffic(n, path):
+    """HBM bytes per Jacobian call from the committed counter summary
     (FETCH_SIZE + WRITE_SIZE per state, scaled by the copy-kernel factor),
     or None when no summary is there."""
     try:
-        for line in open(TRAFFIC_FILE):
+        for line in open(path):
             if line.startswith('traffic_bytes_per_state'):
                 return float(line.split('=')[1]) * n
     except OSError:
@@ -147,15 +151,17 @@ def main():
     d_phi = torch.from_numpy(phi).to(dev)
     d_dphi = torch.empty_like(d_phi)
     d_jac = torch.empty((mech.nnz, n), dtype=torch.float64, device=dev)
-    jk = PyJacobianKernel(n, 0, mech=mech, conp=False, order='F', jac_format=1)
-    # species rates by the mechanism-specialised module when it is built
-    # (__graft_entry__.build()), else by the block kernel; the block
-    # kernel is timed beside it either way
+    # the Jacobian and the species rates by the mechanism-specialised modules
+    # when they are built (__graft_entry__.build()), else by the block
+    # kernels; the block kernels are timed beside them either way
+    jk = PyJacobianKernel(n, 0, mech=mech, conp=False, order='F', jac_format=1, module='auto')
+    jk_block = PyJacobianKernel(n, 0, mech=mech, conp=False, order='F', jac_format=1)
     rk = PySpeciesRatesKernel(n, 0, mech=mech, conp=False, order='F', module='auto')
     rk_block = PySpeciesRatesKernel(n, 0, mech=mech, conp=False, order='F')
 
     for _ in range(args.warmup):
         jk(d_param, d_phi, d_jac)
+        jk_block(d_param, d_phi, d_jac)
         rk(d_param, d_phi, d_dphi)
         rk_block(d_param, d_phi, d_dphi)
     torch.cuda.synchronize()
@@ -175,7 +181,7 @@ def main():
     wall = time.perf_counter() - t0
     jac_ms = float(np.mean([a.elapsed_time(b) for a, b, _ in ev]))
     rates_ms = float(np.mean([b.elapsed_time(c) for _, b, c in ev]))
-    # the block kernel's species rates, outside the timed steps
+    # the block kernels, outside the timed steps
     b0, b1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     b0.record(stream)
     for _ in range(args.steps):
@@ -183,6 +189,12 @@ def main():
     b1.record(stream)
     torch.cuda.synchronize()
     rates_block_ms = b0.elapsed_time(b1) / args.steps
+    b0.record(stream)
+    for _ in range(args.steps):
+        jk_block(d_param, d_phi, d_jac)
+    b1.record(stream)
+    torch.cuda.synchronize()
+    jac_block_ms = b0.elapsed_time(b1) / args.steps
 
     jac_states_s = n / (jac_ms * 1e-3)
     rates_states_s = n / (rates_ms * 1e-3)
@@ -210,15 +222,26 @@ def main():
         'species_rates_states_per_s': rates_states_s,
         'species_rates_kernel': ('pyjac_lane_rates (mechanism-specialised module, one state '
                                  'per lane)' if rk.module else 'rates_block_kernel<1>'),
+        'jacobian_kernel': ('pyjac_ljac_a + pyjac_ljac_b1 (mechanism-specialised module, one '
+                            'state per lane, chunks of 131072 states)' if jk.jac_module
+                            else 'jacobian_block_kernel<2> + entries_to_f2_small_kernel'),
         'jacobian_ms': jac_ms,
+        'jacobian_block_kernel_ms': jac_block_ms,
+        'jacobian_block_kernel_per_s': n / (jac_block_ms * 1e-3),
         'species_rates_ms': rates_ms,
         'species_rates_block_kernel_ms': rates_block_ms,
         'species_rates_block_kernel_states_per_s': n / (rates_block_ms * 1e-3),
         'roofline': {'bound': 'hbm', 'achieved': achieved, 'peak': HBM_PEAK_GBS,
                      'unit': 'GB/s', 'frac': achieved / HBM_PEAK_GBS,
-                     'traffic': measured_traffic(n),
-                     'kernel': 'Jacobian path: jacobian_block_kernel<2> (C-order chunks of '
-                               '65536 states) + entries_to_f2_small_kernel (transpose to F order, on a second stream)',
+                     'traffic': measured_traffic(n, TRAFFIC_FILE_MODULE if jk.jac_module
+                                                 else TRAFFIC_FILE_BLOCK),
+                     'kernel': ('Jacobian path: pyjac_ljac_a (rates, per-reaction values and '
+                                'the temperature column into scratch) + pyjac_ljac_b1 (every '
+                                'entry, F-order lines), per chunk of 131072 states'
+                                if jk.jac_module else
+                                'Jacobian path: jacobian_block_kernel<2> (C-order chunks of '
+                                '65536 states) + entries_to_f2_small_kernel (transpose to F '
+                                'order, on a second stream)'),
                      'algorithmic_bytes_per_state': bytes_per_state},
     }
     if not args.no_surrogate:

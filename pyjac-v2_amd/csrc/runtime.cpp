@@ -891,6 +891,7 @@ int pyjac_hip_load_module(pyjac_handle* h, const char* path) {
         return fail(PYJAC_ERR_IO, "could not load code object %s", path);
     }
     auto reject = [&](int code, const char* why) {
+        (void)hipGetLastError();
         (void)hipModuleUnload(mod);
         return fail(code, "code object %s: %s", path, why);
     };
@@ -898,7 +899,11 @@ int pyjac_hip_load_module(pyjac_handle* h, const char* path) {
     size_t sig_bytes = 0;
     hipFunction_t fn = nullptr;
     unsigned long long sig[4] = {0, 0, 0, 0};
-    if (hipModuleGetGlobal(&sig_ptr, &sig_bytes, mod, "pyjac_ljac_sig") == hipSuccess) {
+    const bool is_jac = hipModuleGetGlobal(&sig_ptr, &sig_bytes, mod, "pyjac_ljac_sig") == hipSuccess;
+    // a failed lookup sets the thread's last HIP error, which the next
+    // launch's error check would report: clear it after every probe
+    (void)hipGetLastError();
+    if (is_jac) {
         // a specialised analytical Jacobian module
         unsigned long long js[7] = {0, 0, 0, 0, 0, 0, 0};
         if (sig_bytes != sizeof(js) || hipMemcpyDtoH(js, sig_ptr, sizeof(js)) != hipSuccess)
@@ -951,8 +956,10 @@ int pyjac_hip_load_module(pyjac_handle* h, const char* path) {
         return PYJAC_OK;
     }
     if (hipModuleGetGlobal(&sig_ptr, &sig_bytes, mod, "pyjac_lane_sig") != hipSuccess ||
-        sig_bytes != sizeof(sig))
+        sig_bytes != sizeof(sig)) {
+        (void)hipGetLastError();
         return reject(PYJAC_ERR_ARG, "not a specialised species-rate or Jacobian module");
+    }
     if (hipMemcpyDtoH(sig, sig_ptr, sizeof(sig)) != hipSuccess)
         return reject(PYJAC_ERR_HIP, "signature unreadable");
     if (sig[1] != (unsigned long long)LANE_VERSION)

@@ -218,8 +218,14 @@ bool jac_module_ready(const pyjac_handle* h) {
     return h->ljac_a && h->ljac_b[ljac_layout(h)] && h->jac_type == PYJAC_JAC_EXACT;
 }
 
+// automatic choice: F order (each wave stores whole lines), and C order for
+// up to 1024 entries per state (measured on MI355X: H2/O2 full and test.inp
+// sparse faster than the block kernel in C order, GRI-Mech 3.0 sparse, 1984
+// entries, 1.4 x slower); variant 31 asks for the module in any layout
 bool use_jac_module(const pyjac_handle* h) {
-    return jac_module_ready(h) && (h->variant[1] == 0 || h->variant[1] == 31);
+    if (!jac_module_ready(h)) return false;
+    if (h->variant[1] == 31) return true;
+    return h->variant[1] == 0 && (h->order == 'F' || nvalues(h) <= 1024);
 }
 
 bool use_block(const pyjac_handle* h, int kernel) {
@@ -2027,7 +2033,9 @@ int pyjac_hip_autotune(pyjac_handle* h, int kernel, size_t n, const double* d_pa
     }
     h->occ[kernel] = best.occ;
     if (kernel == 1) h->fchunk_pref = best_chunk;
-    if (best.v == 30 || best.v == 31) {
+    if (best.v == 31) {
+        h->variant[kernel] = 31;  // the Jacobian module, whatever the layout
+    } else if (best.v == 30) {
         h->variant[kernel] = 0;  // the loaded module is the automatic choice
     } else if (best.v > 20) {
         // automatic choice only when the dense accumulator does not fit

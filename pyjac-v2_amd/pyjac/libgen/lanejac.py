@@ -65,6 +65,7 @@ TARGET_HIP = r'''// generated by pyjac.libgen.lanejac -- mechanism-specialised a
 #define PJ_OPQ(x) __asm__ volatile("" : "+s"(x))
 #define PJ_SS %d
 #define PJ_LANES 64
+#define PJ_NTST(p, v) __builtin_nontemporal_store((double)(v), (p))
 '''
 
 TARGET_HOST = r'''// generated by pyjac.libgen.lanejac -- host build of the specialised Jacobian
@@ -76,6 +77,7 @@ TARGET_HOST = r'''// generated by pyjac.libgen.lanejac -- host build of the spec
 static long pj_ss = 1;
 #define PJ_SS pj_ss
 #define PJ_LANES 1
+#define PJ_NTST(p, v) (*(p) = (v))
 static inline double __dadd_rn(double a, double b) { return a + b; }
 static inline double __dmul_rn(double a, double b) { return a * b; }
 '''
@@ -231,8 +233,14 @@ def _nu(nr, np_):
 class LaneJacSource(lanegen.LaneRatesSource):
     """The generated Jacobian source for one mechanism's tables."""
 
-    def __init__(self, mt):
+    def __init__(self, mt, nt_store=True, u_lds=True):
         super(LaneJacSource, self).__init__(mt)
+        # the entries are stored non-temporally (they are not read again, and
+        # the state that the columns reload stays in L2); the bath-gas terms
+        # of the rows live in LDS beside the Gibbs terms instead of registers
+        self.nt_store = nt_store
+        self.u_lds = u_lds
+        self._nU = 0
         A, Ns, Nr = self.A, self.Ns, self.Nr
         self.ns = Ns - 1
         self.NN = Ns + 1
@@ -600,6 +608,8 @@ class LaneJacSource(lanegen.LaneRatesSource):
         return out
 
     def _store(self, name, i, val):
+        if self.nt_store:
+            return '        PJ_NTST(PJ_SCP(%d), %s);' % (self.sv[(name, i)], val)
         return '        SC(%d) = %s;' % (self.sv[(name, i)], val)
 
     def _reaction(self, r):
@@ -664,7 +674,8 @@ class LaneJacSource(lanegen.LaneRatesSource):
                'PJ_FN void ja_state(long s, long sl, const double* __restrict__ param,',
                '                    const double* __restrict__ phi, double* __restrict__ scr,',
                '                    double* __restrict__ bsh, int ln, long sN, long sI) {',
-               '#define SC(v) scr[(long)(v) * PJ_SS + sl]',
+               '#define SC(v) (*PJ_SCP(v))',
+               '#define PJ_SCP(v) (scr + (long)(v) * PJ_SS + sl)',
                '#define BS(k) bsh[(k) * PJ_LANES + ln]',
                '    const double* ps = phi + s * sN;',
                '    const double T = ps[0], x1 = ps[sI], pv = param[s];',
@@ -760,6 +771,7 @@ class LaneJacSource(lanegen.LaneRatesSource):
         out.append(self._store('JTT', 0, 'JTT'))
         out.append(self._store('JET', 0, 'JET'))
         out.append('#undef SC')
+        out.append('#undef PJ_SCP')
         out.append('#undef LA')
         out.append('#undef BS')
         out.append('}')
@@ -801,9 +813,12 @@ class LaneJacSource(lanegen.LaneRatesSource):
                                'sparsity pattern' % (row, col))
         self.entries.append((row, col))
         NN = self.NN
-        out.append('%so[(long)PJ_POS(%d, %d, %d, %d) * sI] = %s;'
-                   % (indent, self._crs[(row, col)], self._ccs[(row, col)], row * NN + col,
-                      col * NN + row, val))
+        addr = 'o + (long)PJ_POS(%d, %d, %d, %d) * sI' % (
+            self._crs[(row, col)], self._ccs[(row, col)], row * NN + col, col * NN + row)
+        if self.nt_store:
+            out.append('%sPJ_STO(%s, %s);' % (indent, addr, val))
+        else:
+            out.append('%s*(%s) = %s;' % (indent, addr, val))
 
     def _sf_sr(self, r, j):
         """S' and S'' of dRopidnj for column species j (create_jacobian.py
@@ -883,6 +898,10 @@ class LaneJacSource(lanegen.LaneRatesSource):
                '// temperature terms are renewed the same way, so that nothing computed or',
                '// loaded for one column is kept in a register for the next',
                '#define PJ_POS(a, b, c, d) (L == 0 ? (a) : L == 1 ? (b) : L == 2 ? (c) : (d))',
+               '// F-order entries are whole lines written once: stored non-temporally; in C',
+               '// order a lane fills its own state\'s lines entry by entry, and plain stores',
+               '// let the L2 merge them',
+               '#define PJ_STO(p, v) do { if (L & 1) PJ_NTST(p, v); else *(p) = (v); } while (0)',
                'template <int L>',
                'PJ_FN void jb_state(long s, long sl, const double* __restrict__ param,',
                '                    const double* __restrict__ phi, double* __restrict__ scr,',
@@ -1018,8 +1037,13 @@ class LaneJacSource(lanegen.LaneRatesSource):
             for k, nr, np_ in self.slots[r]:
                 if k != ns and _nu(nr, np_) != 0.0:
                     U.setdefault(k, []).append('%s * %s' % (lit(_nu(nr, np_) * dci), G(r)))
+        uname = {}
+        for i, k in enumerate(sorted(U)):
+            uname[k] = ('bsh[(%d) * PJ_LANES + ln]' % (Ns + i)) if self.u_lds else 'U%d' % k
+        self._nU = len(U) if self.u_lds else 0
         if U:
-            out.append('    double %s;' % ', '.join('U%d' % k for k in sorted(U)))
+            if not self.u_lds:
+                out.append('    double %s;' % ', '.join('U%d' % k for k in sorted(U)))
             out.append('    {')
             out.append('        const double Tcc = Tc, logTc = logT, Tinvc = Tinv;')
             out.append('        (void)Tcc; (void)logTc; (void)Tinvc;')
@@ -1031,7 +1055,7 @@ class LaneJacSource(lanegen.LaneRatesSource):
                 out += self._kf_kr(r, True)
                 out.append('        const double Dns%d = %s;' % (r, expr))
             for k in sorted(U):
-                out.append('        U%d = %s;' % (k, ' + '.join(U[k])))
+                out.append('        %s = %s;' % (uname[k], ' + '.join(U[k])))
             out.append('    }')
 
         # ---- species columns
@@ -1071,8 +1095,11 @@ class LaneJacSource(lanegen.LaneRatesSource):
                 terms.append((r, Sf, Sr))
             sp = set()
             rows = set()
-            for r, _, _ in terms:
-                sp.update(k for k, _, _ in self.slots[r])
+            for r, Sf, Sr in terms:
+                # only the concentrations the column's products use
+                for expr in (Sf, Sr if self.rev[r] else None):
+                    if expr is not None:
+                        sp.update(int(x) for x in re.findall(r'\bcg(\d+)\b', expr))
                 rows.update(k for k, nr, np_ in self.slots[r] if k != ns and _nu(nr, np_) != 0.0)
             for r, dci in thd_cols.get(j, []):
                 rows.update(k for k, nr, np_ in self.slots[r] if k != ns and _nu(nr, np_) != 0.0)
@@ -1097,7 +1124,7 @@ class LaneJacSource(lanegen.LaneRatesSource):
             pins.append('dcp%d' % j)
             comp = ['    // column %d' % col]
             for k in rows:
-                comp.append('    double a%d_%d = %s;' % (j, k, 'U%d' % k if k in U else '0.0'))
+                comp.append('    double a%d_%d = %s;' % (j, k, uname[k] if k in U else '0.0'))
             comp.append('    double jt%d, je%d;' % (j, j))
             comp.append('    {')
             comp.append('        double Tcc = Tc, logTc = logT, Tinvc = Tinv;')
@@ -1170,13 +1197,15 @@ class LaneJacSource(lanegen.LaneRatesSource):
         for row in range(NN):
             for col in range(NN):
                 if (row, col) not in self.pattern:
-                    out.append('        o[(long)(L == 2 ? %d : %d) * sI] = 0.0;'
-                               % (row * NN + col, col * NN + row))
+                    addr = 'o + (long)(L == 2 ? %d : %d) * sI' % (row * NN + col, col * NN + row)
+                    out.append(('        PJ_STO(%s, 0.0);' if self.nt_store else '        *(%s) = 0.0;')
+                               % addr)
         out.append('    }')
         out.append('#undef LD')
         out.append('#undef BS')
         out.append('}')
         out.append('#undef PJ_POS')
+        out.append('#undef PJ_STO')
         return out
 
     def source(self, target='hip', kernels='ab'):
@@ -1234,7 +1263,7 @@ class LaneJacSource(lanegen.LaneRatesSource):
                         '    long n, long s0, const double* __restrict__ param, const double* __restrict__ phi,',
                         '    double* __restrict__ scr, double* __restrict__ jac, long sN, long sI,',
                         '    long sS, long jN, int conp) {',
-                        '    __shared__ double bsh[%d * %d];' % (Ns, BLOCK),
+                        '    __shared__ double bsh[%d * %d];' % (Ns + self._nU, BLOCK),
                         '    const long sl = (long)blockIdx.x * %d + threadIdx.x;' % BLOCK,
                         '    if (sl < n) jb_state<%d>(s0 + sl, sl, param, phi, scr, jac, bsh, threadIdx.x,'
                         % L,
@@ -1244,7 +1273,7 @@ class LaneJacSource(lanegen.LaneRatesSource):
         else:
             out += ['extern "C" void pyjac_ljac_host(long n, const double* param, const double* phi,',
                     '    double* scr, double* jac, int layout, long sN, long sI, long jN, int conp) {',
-                    '    double bsh[%d];' % Ns,
+                    '    double bsh[%d];' % (Ns + self._nU),
                     '    pj_ss = n;',
                     '    for (long s = 0; s < n; ++s) {',
                     '        if (conp) ja_state<1>(s, s, param, phi, scr, bsh, 0, sN, sI);',

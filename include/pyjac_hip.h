@@ -316,7 +316,10 @@ size_t pyjac_hip_required_memory(const pyjac_handle *h);
    20 + W (Jacobian) block-per-state kernels that gather the Jacobian in
    device memory (the automatic choice when the dense accumulator does not
    fit in LDS), 30 (species rates) the mechanism-specialised module loaded
-   by pyjac_hip_load_module (the automatic choice once loaded).
+   by pyjac_hip_load_module (the automatic choice once loaded), 31
+   (Jacobian) the mechanism-specialised Jacobian module (the automatic
+   choice once loaded for F order, and for C order up to 1024 entries per
+   state).
    PYJAC_ERR_UNSUPPORTED when this mechanism cannot run the variant (its
    arrays exceed the variant's LDS, or no module is loaded). */
 int pyjac_hip_set_variant(pyjac_handle *h, int kernel, int variant);
@@ -334,8 +337,23 @@ int pyjac_hip_get_variant(const pyjac_handle *h, int kernel);
  * tables (PYJAC_ERR_ARG) or a file it cannot load (PYJAC_ERR_IO).  Once
  * loaded, the handle's species rates use it (variant 30, every data order and
  * both CONP / CONV), the finite-difference Jacobian's perturbed states
- * included; the analytical Jacobian and chem utils are unchanged.
- * tables_hash gives the fingerprint of a tables struct.
+ * included.
+ *
+ * pyjac.libgen.lanejac generates the analytical Jacobian the same way
+ * (create_jacobian.py, generated per mechanism by the reference): one state
+ * per lane, a kernel that evaluates the rates, the temperature column and
+ * the per-reaction values of the other columns into a scratch array, and a
+ * kernel per output layout that stores every entry, column by column.  Such
+ * a module is five code objects (the first kernel, and the store kernel of
+ * sparse C, sparse F, full C and full F); load_module attaches each one
+ * after the same fingerprint check and a check that the module's entry list
+ * is the handle's sparsity pattern (an approximate-Jacobian handle refuses
+ * it).  With the first kernel and the store kernel of the handle's layout
+ * loaded, the handle's analytical Jacobian uses the module (variant 31) for
+ * F order, and for C order up to 1024 entries per state; its scratch array
+ * (about 5.4 KB per GRI state, for a fixed chunk of 131,072 states) is allocated
+ * on the first call.  pyjac_hip_jacobian_dev_noalloc keeps the block
+ * kernels.  tables_hash gives the fingerprint of a tables struct.
  */
 int pyjac_hip_tables_hash(const pyjac_mech_tables *tables, unsigned long long *out);
 int pyjac_hip_load_module(pyjac_handle *h, const char *path);

@@ -1509,7 +1509,13 @@ int pyjac_hip_jacobian_dev_noalloc(pyjac_handle* h, size_t n, const double* para
     hipStream_t s = (hipStream_t)stream;
     double* buf = nullptr;
     const size_t cap = pjk_host::NOALLOC_SCRATCH_BYTES / sizeof(double);
-    if (use_block(h, 1)) {
+    // the specialised module's scratch array (a fixed stride of states per
+    // chunk) does not fit the fixed area: with a module loaded this entry
+    // point runs the block kernels the handle would run without it
+    const int v1 = h->variant[1];
+    const bool block = use_block(h, 1) ||
+                       (use_jac_module(h) && h->block_ok_jac && (v1 == 0 || v1 == 31));
+    if (block) {
         if (h->order == 'C') {
             HIP_TRY(pjk_host::launch_jacobian_block(h->m, h->d_args[1], h->lds[1],
                                                     block_wps(h, 1), h->conp, 'C', (long)n,

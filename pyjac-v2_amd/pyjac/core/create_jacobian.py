@@ -13,6 +13,12 @@ this writes into ``build_path``
 * the mechanism (and thermo) file it was made from, and ``pyjac_hip.json``
   with the options;
 * ``libpyjac_hip.so``, copied from the package build (built on demand);
+* the mechanism-specialised code objects (``pyjac.libgen.lanegen`` species
+  rates, ``pyjac.libgen.lanejac`` analytical Jacobian: the per-mechanism
+  code the reference generates, compiled for gfx950), for mechanisms of up to
+  64 species, which the wrapper module's kernels attach; copied from the
+  package's cache when already built for these tables, else compiled
+  (minutes for GRI-Mech 3.0; ``specialise=False`` skips them);
 * ``tables.pjt`` (the mechanism tables for C and C++ callers),
   ``pyjac_hip_main.cfg`` and the executable ``pyjac_hip_main``, the
   counterpart of the reference's generated main (kernel.cpp.in:645-748):
@@ -53,7 +59,8 @@ mech = _MechTables.from_file(
     last_spec=_opts['last_spec'], rate_spec=_opts['rate_spec'], jac_type=_opts['jac_type'])
 globals().update(_bind(mech, conp=_opts['conp'], order=_opts['order'],
                        jac_format=_opts['jac_format'], jac_type=_opts['jac_type'],
-                       fd_mode=_opts.get('fd_mode', 0), fd_order=_opts.get('fd_order', 1)))
+                       fd_mode=_opts.get('fd_mode', 0), fd_order=_opts.get('fd_order', 1),
+                       module_dir=_here))
 species_names = list(mech.species_names)
 reaction_strings = list(mech.reaction_strings)
 '''
@@ -65,7 +72,7 @@ def create_jacobian(lang, mech_name=None, therm_name=None, build_path='./out/',
                     jac_type=JacobianType.exact, jac_format=JacobianFormat.sparse,
                     fd_order=1, fd_mode=FiniteDifferenceMode.forward,
                     data_filename='data.bin', for_validation=False,
-                    build_library=True, **ignored):
+                    build_library=True, specialise=True, **ignored):
     """Write the HIP kernel module for one mechanism; returns its path."""
     if lang != 'hip':
         raise NotImplementedError('this build provides the hip target only')
@@ -113,6 +120,28 @@ def create_jacobian(lang, mech_name=None, therm_name=None, build_path='./out/',
         main = os.path.join(os.path.dirname(LIB_PATH), MAIN_NAME)
         if os.path.exists(main):
             shutil.copy2(main, os.path.join(build_path, MAIN_NAME))
+    if specialise:
+        _specialise(mech, build_path, jac_type)
     logging.getLogger(__name__).info('wrote %s (%d species, %d reactions)', path, mech.Ns,
                                      mech.Nr)
     return path
+
+
+def _specialise(mech, build_path, jac_type):
+    """The mechanism-specialised modules of these tables into build_path:
+    copied from the package's cache, or compiled there."""
+    from ..libgen import lanegen, lanejac
+    log = logging.getLogger(__name__)
+    if mech.Ns > min(lanegen.MAX_SPECIES, lanejac.MAX_SPECIES):
+        log.info('%d species: no specialised modules (the table-driven kernels run)', mech.Ns)
+        return
+    jobs = [(lanegen, [lanegen.module_path(mech)])]
+    if jac_type == JacobianType.exact:
+        jobs.append((lanejac, lanejac.part_paths(lanejac.module_path(mech))))
+    for gen, cached in jobs:
+        dest = os.path.join(build_path, os.path.basename(cached[0]))
+        if all(os.path.exists(p) for p in cached):
+            for p in cached:
+                shutil.copy2(p, os.path.join(build_path, os.path.basename(p)))
+        else:
+            gen.build(mech, dest)

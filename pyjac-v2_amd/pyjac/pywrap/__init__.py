@@ -166,8 +166,9 @@ class _KernelBase(object):
         self.module = None
         self.jac_module = None
         module = kwargs.pop('module', None)
+        module_dir = kwargs.pop('module_dir', None)
         if module is not None:
-            self.load_module(module)
+            self.load_module(module, module_dir)
         if self._variant_slot <= 1:
             self._apply_tuned()
         if not kwargs.pop('testing_only_skip_compilation', False):
@@ -179,7 +180,7 @@ class _KernelBase(object):
             self.lib.pyjac_hip_destroy(h)
             self._h = None
 
-    def load_module(self, module='auto'):
+    def load_module(self, module='auto', directory=None):
         """Attach mechanism-specialised code objects: a species-rate module
         (``pyjac.libgen.lanegen``) or an analytical Jacobian module
         (``pyjac.libgen.lanejac``), by path, or 'auto' for the cached
@@ -188,15 +189,17 @@ class _KernelBase(object):
         compiled here): the species-rate module always, the Jacobian module
         for a Jacobian kernel of the exact Jacobian.  The species rates (and
         the finite-difference Jacobian's perturbed states) then run on the
-        first, the analytical Jacobian on the second.  Returns the species-
-        rate module's path (the Jacobian module's is ``jac_module``), or None
-        when 'auto' finds neither."""
+        first, the analytical Jacobian on the second.  'auto' looks in
+        ``directory`` (a ``pyjac --lang hip`` build folder) when given, else in
+        the package's cache.  Returns the species-rate module's path (the
+        Jacobian module's is ``jac_module``), or None when 'auto' finds
+        neither."""
         if module == 'auto':
             from ..libgen import lanegen, lanejac
-            rates = lanegen.ensure_module(self.mech, build_missing=False)
+            rates = lanegen.ensure_module(self.mech, build_missing=False, directory=directory)
             jac = None
             if self._variant_slot == 1 and self.jac_type == JacobianType.exact:
-                jac = lanejac.ensure_module(self.mech, build_missing=False)
+                jac = lanejac.ensure_module(self.mech, build_missing=False, directory=directory)
             if rates:
                 check(self.lib.pyjac_hip_load_module(self._h, os.fsencode(rates)))
             if jac:
@@ -555,13 +558,16 @@ PyChem_UtilsKernel = PyChemUtilsKernel
 
 
 def bind(mech, conp=True, order='C', jac_format=JacobianFormat.sparse,
-         jac_type=JacobianType.exact, fd_mode=0, fd_order=1):
+         jac_type=JacobianType.exact, fd_mode=0, fd_order=1, module_dir=None):
     """Kernel classes with the reference's constructor signature
     ``(problem_size, work_size, do_not_compile=False)`` bound to one
     mechanism and option set -- what the reference's generated
-    ``pyjac_<lang>`` module provides."""
+    ``pyjac_<lang>`` module provides.  With ``module_dir`` (a build folder
+    holding the mechanism-specialised modules) the kernels attach them."""
     opts = dict(mech=mech, conp=conp, order=order, jac_format=jac_format,
                 jac_type=jac_type)
+    if module_dir:
+        opts.update(module='auto', module_dir=module_dir)
 
     def make(base):
         class Bound(base):

@@ -37,7 +37,7 @@ def test_parser_matches_reference_flags():
 def test_generated_module(tmp_path):
     mech, therm = mechanism_files('gri30')
     path = create_jacobian('hip', mech, therm, build_path=str(tmp_path), data_order='F',
-                           conp=False, build_library=False)
+                           conp=False, build_library=False, specialise=False)
     mod = _load(path)
     assert mod.species_names[-1] == 'N2'
     assert len(mod.reaction_strings) == 325 and mod.mech.nnz == 1984
@@ -54,6 +54,21 @@ def test_command_line(tmp_path):
     assert (tmp_path / 'pyjac_hip.py').exists() and (tmp_path / 'pyjac_hip.json').exists()
 
 
+def test_build_folder_holds_specialised_modules(tmp_path):
+    """The build folder carries the mechanism-specialised code objects (the
+    per-mechanism code the reference generates): the species-rate module and
+    the five parts of the Jacobian module, named by the tables' fingerprint."""
+    from pyjac.core.mech_tables import MechTables
+    from pyjac.libgen import lanegen, lanejac
+    mech, therm = mechanism_files('h2o2')
+    create_jacobian('hip', mech, therm, build_path=str(tmp_path), data_order='C', conp=True,
+                    build_library=False)
+    mt = MechTables.from_file(mech, therm)
+    assert (tmp_path / os.path.basename(lanegen.module_path(mt))).exists()
+    for part in lanejac.part_paths(lanejac.module_path(mt)):
+        assert (tmp_path / os.path.basename(part)).exists()
+
+
 @pytest.mark.gpu
 def test_generated_module_runs(tmp_path):
     import oracle
@@ -65,7 +80,10 @@ def test_generated_module_runs(tmp_path):
     n = 100
     param, phi = sample_states(mod.mech, n, seed=5, conp=True)
     jac = np.zeros((n, mod.mech.NN, mod.mech.NN))
-    mod.PyJacobianKernel(n, 0)(param, phi, jac)
+    k = mod.PyJacobianKernel(n, 0)
+    # the kernels attach the specialised modules of the build folder
+    assert k.jac_module and os.path.dirname(k.jac_module) == str(tmp_path)
+    k(param, phi, jac)
     ref = oracle.jacobian(mod.mech, True, param, phi)
     fro = np.linalg.norm((jac - ref).reshape(n, -1), axis=1) / np.linalg.norm(ref.reshape(n, -1), axis=1)
     assert fro.max() <= 1e-8

@@ -69,6 +69,23 @@ def test_noalloc_matches_regular(mechs, name, order, fmt, sizes):
         _run_both(k, mech, n, order, fmt, seed=60 + i)
 
 
+def test_noalloc_with_jacobian_module(mechs):
+    """With the mechanism-specialised Jacobian module loaded, the regular
+    entry point runs the module and the one that allocates nothing runs the
+    block kernels (the module's scratch array does not fit the fixed area):
+    the two agree, twice in a row, the second call smaller."""
+    from pyjac.libgen import lanejac
+    mech = mechs('gri30')
+    path = lanejac.ensure_module(mech, build_missing=False)
+    if not path:
+        pytest.skip('Jacobian module not built')
+    k = PyJacobianKernel(9000, 0, mech=mech, conp=False, order='F', jac_format=1, module=path)
+    assert k.jac_module
+    for i, n in enumerate((9000, 1001)):
+        print('no-alloc with module', n, flush=True)
+        _run_both(k, mech, n, 'F', 1, seed=80 + i)
+
+
 def test_noalloc_refuses_oversized(mechs):
     """A full-format F-order Jacobian of 500 species needs 2 MB per state
     of chunk buffer: 64 states exceed the 64 MiB area."""

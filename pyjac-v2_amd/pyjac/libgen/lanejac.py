@@ -310,12 +310,28 @@ class LaneJacSource(lanegen.LaneRatesSource):
                 sr.append(_powi(self.cgfmt % k, nr))
         return ' * '.join(sf), ' * '.join(sr)
 
+    def _db_species(self, r):
+        """Species whose dB/dT the reaction's temperature term reads."""
+        if not self.rev[r] or not (int(self.A['simple_mask'][r]) >= 0 or self.table_kf[r]):
+            return []
+        return [k for k, nr, np_ in self.slots[r] if _nu(nr, np_) != 0.0]
+
+    def _db_loads(self, r):
+        """(loads, pins) of the reaction's dB/dT values: issued before the
+        previous reaction's block and pinned after it, so that waiting for
+        them does not wait for that block's stores."""
+        ks = self._db_species(r)
+        loads = ['    PJ_OPAQUE(oa);']
+        loads += ['    double dbr%d_%d = LA(%d);' % (r, k, self.sv[('DB', k)]) for k in ks]
+        pins = ['    PJ_OPAQUE(dbr%d_%d);' % (r, k) for k in ks]
+        return loads, pins
+
     def _dB(self, r):
         terms = []
         for k, nr, np_ in self.slots[r]:
             nu = _nu(nr, np_)
             if nu != 0.0:
-                terms.append('%s * LA(%d)' % (lit(nu), self.sv[('DB', k)]))
+                terms.append('%s * dbr%d_%d' % (lit(nu), r, k))
         return ' + '.join(terms) if terms else '0.0'
 
     def _dropi_dT(self, r, dkf):
@@ -623,10 +639,7 @@ class LaneJacSource(lanegen.LaneRatesSource):
         # a fresh opaque base for the reaction's scratch loads, so that loads
         # of one value by several reactions are not merged into one long-lived
         # register
-        # a fresh opaque offset for the reaction's scratch loads, so that loads
-        # of one value by several reactions are not merged into one long-lived
-        # register
-        body = out[:1] + ['        PJ_OPAQUE(oa);'] + out[1:-1]
+        body = out[:-1]
         if self.table_kf[r]:
             body.append(self._store('KF', r, 'kf'))
         if self.thd[r] >= 0:
@@ -708,8 +721,17 @@ class LaneJacSource(lanegen.LaneRatesSource):
             out.append(self._store('DB', k, 'dgibbs(Tc, Tinv, %s)' % self._coeffs(k, 6, 'dq')))
         out.append('    double %s;' % ', '.join('w%d = 0.0' % k for k in range(Ns)))
         out.append('    double %s;' % ', '.join('jt%d = 0.0' % k for k in range(Ns - 1)))
+        # reaction r + 1's scratch loads (through a fresh opaque offset, so that
+        # loads of one value by several reactions are not merged into one
+        # long-lived register) are issued before reaction r's block and pinned
+        # after it
+        loads, pins = self._db_loads(0)
+        out += loads + pins
         for r in range(self.Nr):
+            nxt = self._db_loads(r + 1) if r + 1 < self.Nr else ([], [])
+            out += nxt[0]
             out += self._reaction(r)
+            out += nxt[1]
         # energies, heat capacities, temperature rate; the temperature column's
         # corner entries; the per-species terms of the other rows
         out += ['    double Tq = Tc;', '    PJ_OPAQUE(Tq);']

@@ -964,43 +964,70 @@ class LaneJacSource(lanegen.LaneRatesSource):
             out.append('    const double ew%d = LD(%d);' % (k, self.sv[('EW', k)]))
 
         # ---- the temperature column (kernel A's) and the extra-variable
-        # column, the row sums of the per-reaction contributions
+        # column.  Scratch values are loaded in batches, each batch pinned by
+        # an empty asm statement before its first use, so that the loads of a
+        # batch are in flight together
+        def batched(names_vals, consume, indent='    ', size=16):
+            """Loads `name = value` in batches of `size`, pinned, each batch
+            followed by consume(batch) statements."""
+            body = []
+            for b0 in range(0, len(names_vals), size):
+                batch = names_vals[b0:b0 + size]
+                body.append('%sPJ_OPAQUE(ob);' % indent)
+                body += ['%sdouble %s = %s;' % (indent, nm, val) for nm, val in batch]
+                body += ['%sPJ_OPAQUE(%s);' % (indent, nm) for nm, _ in batch]
+                body += consume(batch)
+            return body
+
         out.append('    {  // column 0')
-        for i in range(Ns - 1):
-            if (i + 2, 0) in self.pattern:
-                self._entry(i + 2, 0, LD(self.sv[('JT', i)]), out)
-        self._entry(0, 0, LD(self.sv[('JTT', 0)]), out)
-        self._entry(1, 0, LD(self.sv[('JET', 0)]), out)
+        col0 = [(i + 2, 'jt_%d' % i, LD(self.sv[('JT', i)])) for i in range(Ns - 1)
+                if (i + 2, 0) in self.pattern]
+        col0 += [(0, 'jt_T', LD(self.sv[('JTT', 0)])), (1, 'jt_E', LD(self.sv[('JET', 0)]))]
+        rowof = {nm: row for row, nm, _ in col0}
+
+        def store0(batch):
+            body = []
+            for nm, _ in batch:
+                self._entry(rowof[nm], 0, nm, body)
+            return body
+        out += batched([(nm, val) for _, nm, val in col0], store0, '        ')
         out.append('    }')
-        rows_of = [[] for _ in range(Ns)]
-        for r in range(self.Nr):
-            for k, nr, np_ in self.slots[r]:
-                if k != ns and _nu(nr, np_) != 0.0:
-                    rows_of[k].append((r, _nu(nr, np_)))
         for mode in (1, 0):
             out.append('    %s {  // column 1' % ('if (conp)' if mode else 'else'))
+            rxn = [r for r in range(self.Nr) if mode or self.ee_conv[r]]
+            rows = sorted({k for r in rxn for k, nr, np_ in self.slots[r]
+                           if k != ns and _nu(nr, np_) != 0.0})
+            for i in rows:
+                out.append('        double e_%d = 0.0;' % i)
+
+            def add(batch):
+                body = []
+                for nm, _ in batch:
+                    r = int(nm[3:])
+                    for k, nr, np_ in self.slots[r]:
+                        if k != ns and _nu(nr, np_) != 0.0:
+                            body.append('        e_%d = e_%d + %s * %s;' % (k, k, lit(_nu(nr, np_)), nm))
+                return body
+            # each value once, its rows' sums in reaction order
+            out += batched([('ee_%d' % r, LD(self.sv[('EE', r)])) for r in rxn], add, '        ')
+            if mode:
+                out += batched([('wd_%d' % i, LD(self.sv[('WD', i)])) for i in range(Ns - 1)],
+                               lambda b: [], '        ')
             out.append('        double specsum = 0.0, esum = 0.0;')
             for i in range(Ns - 1):
-                terms = ['%s * %s' % (lit(nu), LD(self.sv[('EE', r)])) for r, nu in rows_of[i]
-                         if mode or self.ee_conv[r]]
                 mwf = lit(1.0 + -1.0 * float(A['mw_post'][i]))
-                wd = LD(self.sv[('WD', i)])
-                out.append('        {')
-                out.append('            PJ_OPAQUE(ob);')
-                if terms:
-                    out.append('            const double J = %s;' % ' + '.join(terms))
-                    self._entry(i + 2, 1, 'J', out, '            ')
+                if i in rows:
+                    self._entry(i + 2, 1, 'e_%d' % i, out)
                     if mode:
-                        out.append('            specsum = specsum + ew%d * (J + -1.0 * %s);' % (i, wd))
+                        out.append('        specsum = specsum + ew%d * (e_%d + -1.0 * wd_%d);' % (i, i, i))
                     else:
-                        out.append('            specsum = specsum + ew%d * J;' % i)
-                    out.append('            esum = esum + %s * J;' % mwf)
+                        out.append('        specsum = specsum + ew%d * e_%d;' % (i, i))
+                    out.append('        esum = esum + %s * e_%d;' % (mwf, i))
                 else:
                     if (i + 2, 1) in self.pattern:
-                        self._entry(i + 2, 1, '0.0', out, '            ')
+                        self._entry(i + 2, 1, '0.0', out)
                     if mode:
-                        out.append('            specsum = specsum + ew%d * (-1.0 * %s);' % (i, wd))
-                out.append('        }')
+                        out.append('        specsum = specsum + ew%d * (-1.0 * wd_%d);' % (i, i))
             if mode:
                 out.append('        const double JTE = (dTdt * %s + -1.0 * specsum) * (1.0 / (cptot * V));'
                            % LD(self.sv[('DTSUM2', 0)]))

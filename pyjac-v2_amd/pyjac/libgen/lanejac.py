@@ -45,6 +45,7 @@ import numpy as np
 
 from . import lanegen
 from .lanegen import lit, _chain, _powi, tables_hash
+from ..core.chem_model import RU as RU_, PA as PA_
 
 LJAC_VERSION = 1
 KERNEL_A = 'pyjac_ljac_a'
@@ -635,6 +636,12 @@ class LaneJacSource(lanegen.LaneRatesSource):
         accumulators, the rest is stored."""
         out = super(LaneJacSource, self)._reaction(r)
         assert out[-1] == '    }'
+        if self.rev[r]:
+            # the reverse rate constant as kernel B evaluates it: a product
+            # with exp(-B_sum) instead of a division by exp(B_sum)
+            i = [n for n, l in enumerate(out) if l.startswith('        const double kr = ')]
+            assert len(i) == 1
+            out[i[0]] = '        const double kr = %s;' % self._kr_product(r, 'kf')
         out = [re.sub(r'\bb(\d+)\b', r'BS(\1)', l) for l in out]
         # a fresh opaque base for the reaction's scratch loads, so that loads
         # of one value by several reactions are not merged into one long-lived
@@ -862,32 +869,54 @@ class LaneJacSource(lanegen.LaneRatesSource):
         Sr = (' * '.join(sr_terms) if sr_terms else '1.0') if sr0 else None
         return Sf, Sr
 
-    def _kf_kr(self, r, need_kr, indent='        ', kf_loaded=None):
-        """Definitions of kf<r> (and kr<r>) in kernel B: the Arrhenius rate
-        constant recomputed from its literals exactly as kernel A evaluates
-        it (PLOG and Chebyshev rate constants are read back from scratch),
-        and the reverse rate constant from the equilibrium constant of the
-        Gibbs terms, the reaction block's expression
-        (rate_subs.py:1627-1785)."""
+    def _kr_product(self, r, kf, kf_exponent=None):
+        """kr of reversible reaction r as kf (1 / P_sum) exp(-B_sum); with
+        kf_exponent, (1 / P_sum) exp(kf_exponent - B_sum) (kf = exp of it)."""
         A = self.A
+        nb = ['%s * BS(%d)' % (lit(float(nr - np_)), k)
+              for k, nr, np_ in self.slots[r] if np_ - nr != 0]
+        nsum = int(A['nu_sum'][r])
+        pinv = 1.0
+        for _ in range(abs(nsum)):
+            pinv = pinv * (RU_ / PA_ if nsum > 0 else PA_ / RU_)
+        fac = '' if nsum == 0 else '%s * ' % lit(pinv)
+        if kf_exponent is not None:
+            return '%sex(%s)' % (fac, ' + '.join([kf_exponent] + nb))
+        return '%s * %sex(%s)' % (kf, fac, ' + '.join(['0.0'] + nb))
+
+    def _kf_kr(self, r, need_kr, indent='        ', kf_loaded=None, need_kf=True):
+        """Definitions of kf<r> (if need_kf) and kr<r> (if need_kr) in kernel
+        B: the Arrhenius rate constant recomputed from its literals exactly as
+        kernel A evaluates it (PLOG and Chebyshev rate constants are read back
+        from scratch), and the reverse rate constant kf / (P_sum exp(B_sum))
+        of the Gibbs terms (rate_subs.py:1627-1785) evaluated as kf (1 /
+        P_sum) exp(-B_sum), a product instead of a division; where the column
+        needs kr alone and kf is an Arrhenius exponential, the two exponents
+        are added and one exponential taken.  Either form is the quotient to a
+        few units in the last place."""
+        A = self.A
+        exp_form = False
         if self.table_kf[r]:
             kf = kf_loaded or 'LD(%d)' % self.sv[('KF', r)]
         else:
             kf = self._kf(r)
+            si = int(A['simple_mask'][r])
+            exp_form = si >= 0 and int(A['simple_form'][si]) >= 2
             # the column's opaque copies of the temperature terms
             kf = re.sub(r'\bTc\b', 'Tcc', kf)
             kf = re.sub(r'\blogT\b', 'logTc', kf)
             kf = re.sub(r'\bTinv\b', 'Tinvc', kf)
-        out = ['%sconst double kf%d = %s;' % (indent, r, kf)]
-        if need_kr and self.rev[r]:
-            bsum = '0.0'
-            for k, nr, np_ in self.slots[r]:
-                if np_ - nr != 0:
-                    bsum = '%s + %s * BS(%d)' % (bsum, lit(float(np_ - nr)), k)
-            nsum = int(A['nu_sum'][r])
-            pval = 'PA / RU' if nsum > 0 else 'RU / PA'
-            psum = _chain('(%s)' % pval, abs(nsum))
-            out.append('%sconst double kr%d = kf%d / (%s * ex(%s));' % (indent, r, r, psum, bsum))
+        need_kr = need_kr and self.rev[r]
+        out = []
+        if need_kf or not need_kr:
+            out.append('%sconst double kf%d = %s;' % (indent, r, kf))
+        if need_kr:
+            if not need_kf and exp_form:
+                assert kf.startswith('ex(') and kf.endswith(')')
+                kr = self._kr_product(r, None, kf[3:-1])
+            else:
+                kr = self._kr_product(r, 'kf%d' % r if need_kf else '(%s)' % kf)
+            out.append('%sconst double kr%d = %s;' % (indent, r, kr))
         return out
 
     def _cg_defs(self, species, indent='        '):
@@ -1191,7 +1220,8 @@ class LaneJacSource(lanegen.LaneRatesSource):
                     pr.append('-1.0 * %s * %s' % (KR(r), Sr))
                 comp.append('        {')
                 comp += self._kf_kr(r, Sr is not None, '            ',
-                                    'kt%d_%d' % (j, r) if self.table_kf[r] else None)
+                                    'kt%d_%d' % (j, r) if self.table_kf[r] else None,
+                                    need_kf=Sf is not None)
                 expr = ' + '.join(pr)
                 if self.thd[r] >= 0:
                     expr = '(%s) * ci%d_%d' % (expr, j, r)

@@ -44,6 +44,8 @@ import subprocess
 
 import numpy as np
 
+from ..core.chem_model import RU as RU_, PA as PA_
+
 LANE_VERSION = 1
 KERNEL = 'pyjac_lane_rates'
 SIG = 'pyjac_lane_sig'
@@ -137,6 +139,23 @@ def _powi(val, p):
     if p < 0:
         return _chain('(1.0 / %s)' % val, -p)
     return _chain(val, p)
+
+
+def kr_product(slots, nsum, kf, gibbs='b%d', kf_exponent=None):
+    """The reverse rate constant kf / (P_sum exp(B_sum)) of a reversible
+    reaction (rate_subs.py:1627-1785; P_sum = (PA / RU)^nsum, B_sum the
+    stoichiometric sum of the Gibbs terms `gibbs`) as the product kf (1 /
+    P_sum) exp(-B_sum), or, given the exponent of an Arrhenius kf, as (1 /
+    P_sum) exp(kf_exponent - B_sum): the quotient to a few units in the last
+    place, without a division."""
+    nb = ['%s * %s' % (lit(float(nr - np_)), gibbs % k) for k, nr, np_ in slots if np_ - nr != 0]
+    pinv = 1.0
+    for _ in range(abs(nsum)):
+        pinv = pinv * (RU_ / PA_ if nsum > 0 else PA_ / RU_)
+    fac = '' if nsum == 0 else '%s * ' % lit(pinv)
+    if kf_exponent is not None:
+        return '%sex(%s)' % (fac, ' + '.join([kf_exponent] + nb))
+    return '%s * %sex(%s)' % (kf, fac, ' + '.join(['0.0'] + nb))
 
 
 def _arrhenius(form, A, b, Ta):
@@ -407,17 +426,8 @@ class LaneRatesSource(object):
         out.append('        const double rf = %s;' % rf)
         rr = None
         if int(A['rev_mask'][r]) >= 0:
-            terms = ['0.0']
-            for k, nr, np_ in slots:
-                if np_ - nr != 0:
-                    terms.append('%s * b%d' % (lit(float(np_ - nr)), k))
-            bsum = terms[0]
-            for tm in terms[1:]:
-                bsum = '%s + %s' % (bsum, tm)
-            nsum = int(A['nu_sum'][r])
-            pval = 'PA / RU' if nsum > 0 else 'RU / PA'
-            psum = _chain('(%s)' % pval, abs(nsum))
-            out.append('        const double kr = kf / (%s * ex(%s));' % (psum, bsum))
+            out.append('        const double kr = %s;'
+                       % kr_product(slots, int(A['nu_sum'][r]), 'kf'))
             rr = 'kr'
             for k, _, np_ in slots:
                 if np_:

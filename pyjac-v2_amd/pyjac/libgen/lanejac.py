@@ -44,8 +44,7 @@ import subprocess
 import numpy as np
 
 from . import lanegen
-from .lanegen import lit, _chain, _powi, tables_hash
-from ..core.chem_model import RU as RU_, PA as PA_
+from .lanegen import lit, _powi, tables_hash
 
 LJAC_VERSION = 1
 KERNEL_A = 'pyjac_ljac_a'
@@ -636,12 +635,6 @@ class LaneJacSource(lanegen.LaneRatesSource):
         accumulators, the rest is stored."""
         out = super(LaneJacSource, self)._reaction(r)
         assert out[-1] == '    }'
-        if self.rev[r]:
-            # the reverse rate constant as kernel B evaluates it: a product
-            # with exp(-B_sum) instead of a division by exp(B_sum)
-            i = [n for n, l in enumerate(out) if l.startswith('        const double kr = ')]
-            assert len(i) == 1
-            out[i[0]] = '        const double kr = %s;' % self._kr_product(r, 'kf')
         out = [re.sub(r'\bb(\d+)\b', r'BS(\1)', l) for l in out]
         # a fresh opaque base for the reaction's scratch loads, so that loads
         # of one value by several reactions are not merged into one long-lived
@@ -870,19 +863,10 @@ class LaneJacSource(lanegen.LaneRatesSource):
         return Sf, Sr
 
     def _kr_product(self, r, kf, kf_exponent=None):
-        """kr of reversible reaction r as kf (1 / P_sum) exp(-B_sum); with
-        kf_exponent, (1 / P_sum) exp(kf_exponent - B_sum) (kf = exp of it)."""
-        A = self.A
-        nb = ['%s * BS(%d)' % (lit(float(nr - np_)), k)
-              for k, nr, np_ in self.slots[r] if np_ - nr != 0]
-        nsum = int(A['nu_sum'][r])
-        pinv = 1.0
-        for _ in range(abs(nsum)):
-            pinv = pinv * (RU_ / PA_ if nsum > 0 else PA_ / RU_)
-        fac = '' if nsum == 0 else '%s * ' % lit(pinv)
-        if kf_exponent is not None:
-            return '%sex(%s)' % (fac, ' + '.join([kf_exponent] + nb))
-        return '%s * %sex(%s)' % (kf, fac, ' + '.join(['0.0'] + nb))
+        """kr of reversible reaction r, the species-rate module's product form
+        with the Gibbs terms in LDS."""
+        return lanegen.kr_product(self.slots[r], int(self.A['nu_sum'][r]), kf, 'BS(%d)',
+                                  kf_exponent)
 
     def _kf_kr(self, r, need_kr, indent='        ', kf_loaded=None, need_kf=True):
         """Definitions of kf<r> (if need_kf) and kr<r> (if need_kr) in kernel

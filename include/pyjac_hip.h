@@ -317,9 +317,11 @@ size_t pyjac_hip_required_memory(const pyjac_handle *h);
    device memory (the automatic choice when the dense accumulator does not
    fit in LDS), 30 (species rates) the mechanism-specialised module loaded
    by pyjac_hip_load_module (the automatic choice once loaded), 31
-   (Jacobian) the mechanism-specialised Jacobian module (the automatic
-   choice once loaded for F order, and for C order up to 1024 entries per
-   state).
+   (Jacobian) the mechanism-specialised Jacobian module storing the output
+   directly (the automatic choice once loaded in F order), 32 (Jacobian, C
+   order) the module's C-layout store kernel writing lines of a chunk
+   (entry stride = the chunk's states) that a tiled transpose writes out as
+   C-order rows (the automatic choice once loaded in C order).
    PYJAC_ERR_UNSUPPORTED when this mechanism cannot run the variant (its
    arrays exceed the variant's LDS, or no module is loaded). */
 int pyjac_hip_set_variant(pyjac_handle *h, int kernel, int variant);
@@ -349,10 +351,10 @@ int pyjac_hip_get_variant(const pyjac_handle *h, int kernel);
  * after the same fingerprint check and a check that the module's entry list
  * is the handle's sparsity pattern (an approximate-Jacobian handle refuses
  * it).  With the first kernel and the store kernel of the handle's layout
- * loaded, the handle's analytical Jacobian uses the module (variant 31) for
- * F order, and for C order up to 1024 entries per state; its scratch array
- * (about 5.4 KB per GRI state, for a fixed chunk of 131,072 states) is allocated
- * on the first call.  pyjac_hip_jacobian_dev_noalloc keeps the block
+ * loaded, the handle's analytical Jacobian uses the module: variant 31 in F order,
+ * variant 32 in C order; its scratch array (about 5.4 KB per GRI state, for a
+ * fixed chunk of 131,072 states), and in C order a chunk of entries (15.9 KB
+ * per GRI state, sparse), are allocated on the first call.  pyjac_hip_jacobian_dev_noalloc keeps the block
  * kernels.  tables_hash gives the fingerprint of a tables struct.
  */
 int pyjac_hip_tables_hash(const pyjac_mech_tables *tables, unsigned long long *out);

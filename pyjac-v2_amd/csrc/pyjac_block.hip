@@ -1716,6 +1716,34 @@ entries_to_f2_small_kernel(const double* __restrict__ src, long count, int nout,
     }
 }
 
+// The other direction, for the specialised Jacobian module in C order: its
+// store kernel writes a chunk's entries as F-order lines (entry e of chunk
+// state i at src[e * count + i], whole lines per wave) and this kernel writes
+// them out as C-order rows (state s0 + i at dst[(s0 + i) * nout]).  32 x 32
+// tiles through LDS: a wave reads two 256-byte runs of one entry's states and
+// writes two 256-byte runs of one state's entries.  Both arrays are touched
+// once: non-temporal loads and stores.
+__global__ void __launch_bounds__(256)
+entries_to_c_kernel(const double* __restrict__ src, long count, int nout,
+                    double* __restrict__ dst, long s0) {
+    __shared__ double tile[32][33];
+    const long i0 = (long)blockIdx.x * 32;
+    const int e0 = blockIdx.y * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 8 rows per pass
+    for (int r = ty; r < 32; r += 8) {
+        const int e = e0 + r;
+        const long i = i0 + tx;
+        if (e < nout && i < count) tile[r][tx] = __builtin_nontemporal_load(src + (size_t)e * count + i);
+    }
+    __syncthreads();
+    for (int r = ty; r < 32; r += 8) {
+        const long i = i0 + r;
+        const int e = e0 + tx;
+        if (i < count && e < nout)
+            __builtin_nontemporal_store(tile[tx][r], dst + (size_t)(s0 + i) * nout + e);
+    }
+}
+
 // Ragged batch: states of several mechanisms / option sets in one launch.
 // Member i owns workgroups [block0_i, block0_i + 8 ceil(n_i / 8)); inside
 // its range the XCD-aware state order of the single-mechanism launch applies.
@@ -2035,6 +2063,15 @@ static hipError_t launch_jacobian_block_chunk(const DevMech& m, const BlockArgs*
     const long grid = 8 * ((cnt + 7) / 8);
     return launch_jac_kernel(wps, occ, lds, grid, stream, args, conp, cnt, param + s0, phi + s0,
                              out, L, 1);
+}
+
+hipError_t launch_entries_to_c(const double* src, long count, int nout, double* jac, long s0,
+                               hipStream_t stream) {
+    if (count <= 0 || nout <= 0) return hipSuccess;
+    dim3 grid((unsigned)((count + 31) / 32), (unsigned)((nout + 31) / 32));
+    hipLaunchKernelGGL(pjb::entries_to_c_kernel, grid, dim3(256), 0, stream, src, count, nout,
+                       jac, s0);
+    return hipGetLastError();
 }
 
 hipError_t launch_jacobian_block(const DevMech& m, const BlockArgs* args, const BlockLds& BL,

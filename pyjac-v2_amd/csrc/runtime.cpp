@@ -132,6 +132,10 @@ struct pyjac_handle {
     long ljac_nscr = 0, ljac_stride = 0;
     double* d_ljac_scr = nullptr;
     size_t ljac_scr_doubles = 0;
+    // C order through F-order lines (variant 32): one chunk's entries, which
+    // a tiled transpose writes out as C-order rows
+    double* d_ljac_lines = nullptr;
+    size_t ljac_lines_doubles = 0;
     // the stored sparsity pattern, row * NN + col, whatever the format
     std::vector<int> pattern;
 };
@@ -218,14 +222,32 @@ bool jac_module_ready(const pyjac_handle* h) {
     return h->ljac_a && h->ljac_b[ljac_layout(h)] && h->jac_type == PYJAC_JAC_EXACT;
 }
 
-// automatic choice: F order (each wave stores whole lines), and C order for
-// up to 1024 entries per state (measured on MI355X: H2/O2 full and test.inp
-// sparse faster than the block kernel in C order, GRI-Mech 3.0 sparse, 1984
-// entries, 1.4 x slower); variant 31 asks for the module in any layout
+// C order through lines (variant 32): the handle's own store kernel, whose
+// entry positions are those of C order, writes a chunk's entries with the
+// strides of F-order lines (entry stride = the chunk's states), and a tiled
+// transpose writes the lines out as C-order rows
+bool ljac_lines_ready(const pyjac_handle* h) {
+    return h->order == 'C' && jac_module_ready(h);
+}
+
+// the module runs through lines and the transpose: asked for (variant 32),
+// or the automatic choice in C order, where a lane storing its own state's
+// entries leaves each store scattered over 64 rows (measured on MI355X,
+// 262,144 states: GRI-Mech 3.0 sparse 4.7 ms through lines, 9.5 ms stored
+// directly, 6.8 ms for the block kernel; H2/O2 full, 65,536 states, 0.083
+// against 0.143 ms)
+bool ljac_through_lines(const pyjac_handle* h) {
+    if (!ljac_lines_ready(h)) return false;
+    return h->variant[1] == 32 || h->variant[1] == 0;
+}
+
+// automatic choice once loaded: F order (each wave stores whole lines), C
+// order through lines; variant 31 asks for the store kernel of the handle's
+// layout writing the output directly, in either order
 bool use_jac_module(const pyjac_handle* h) {
+    if (ljac_through_lines(h)) return true;
     if (!jac_module_ready(h)) return false;
-    if (h->variant[1] == 31) return true;
-    return h->variant[1] == 0 && (h->order == 'F' || nvalues(h) <= 1024);
+    return h->variant[1] == 31 || (h->variant[1] == 0 && h->order == 'F');
 }
 
 bool use_block(const pyjac_handle* h, int kernel) {
@@ -423,7 +445,7 @@ int prepare_lane(pyjac_handle* h, int kernel, long n) {
 // the specialised module's kernel: one state per lane, 64 per workgroup;
 // entry i of state s at [s * sN + i * sI]
 constexpr int LANE_VERSION = 1;
-constexpr int LJAC_VERSION = 1;
+constexpr int LJAC_VERSION = 2;
 constexpr int LANE_BLOCK = 64;
 
 int launch_module(pyjac_handle* h, long n, const double* param, const double* phi, double* dphi,
@@ -513,22 +535,33 @@ int run_fd_jac(pyjac_handle* h, long n, const double* param, const double* phi, 
 // the specialised Jacobian module: per chunk of at most `ljac_stride`
 // states, kernel A writes the rates' per-reaction and per-species values to
 // the scratch array ([value][state in chunk]) and kernel B stores every entry
-// in the handle's layout; one state per lane, 64 per workgroup
+// at its C- or F-order position p of the handle's layout (jac + (s - jS) * jN
+// + p * jI), or, through lines, at its C-order position in lines of the
+// chunk (jN = 1, jS = s0, jI = the chunk's states) that a tiled transpose
+// writes out as C-order rows; one state per lane, 64 per workgroup
+int grow_buffer(double** buf, size_t* have, size_t need) {
+    if (need <= *have) return PYJAC_OK;
+    if (*buf) (void)hipFree(*buf);
+    *buf = nullptr;
+    *have = 0;
+    HIP_TRY(hipMalloc(buf, need * sizeof(double)));
+    *have = need;
+    return PYJAC_OK;
+}
+
 int launch_jac_module(pyjac_handle* h, long n, const double* param, const double* phi,
                       double* jac, hipStream_t s) {
     const long stride = h->ljac_stride;
-    const size_t need = (size_t)(h->ljac_nscr - 1) * stride + (size_t)std::min(n, stride);
-    if (need > h->ljac_scr_doubles) {
-        if (h->d_ljac_scr) (void)hipFree(h->d_ljac_scr);
-        h->d_ljac_scr = nullptr;
-        h->ljac_scr_doubles = 0;
-        HIP_TRY(hipMalloc(&h->d_ljac_scr, need * sizeof(double)));
-        h->ljac_scr_doubles = need;
-    }
+    const long ne = nvalues(h);
+    const bool lines = ljac_through_lines(h);
+    int rc = grow_buffer(&h->d_ljac_scr, &h->ljac_scr_doubles,
+                  (size_t)(h->ljac_nscr - 1) * stride + (size_t)std::min(n, stride));
+    if (!rc && lines)
+        rc = grow_buffer(&h->d_ljac_lines, &h->ljac_lines_doubles, (size_t)ne * std::min(n, stride));
+    if (rc) return rc;
     const bool F = h->order == 'F';
     const int layout = ljac_layout(h);
     long sN = F ? 1 : h->m.NN, sI = F ? n : 1;
-    long jN = F ? 1 : nvalues(h);
     int conp = h->conp;
     double* scr = h->d_ljac_scr;
     for (long s0 = 0; s0 < n; s0 += stride) {
@@ -536,10 +569,15 @@ int launch_jac_module(pyjac_handle* h, long n, const double* param, const double
         const unsigned grid = (unsigned)((cnt + LANE_BLOCK - 1) / LANE_BLOCK);
         void* aa[] = {&cnt, &s0, (void*)&param, (void*)&phi, &scr, &sN, &sI, (void*)&stride, &conp};
         HIP_TRY(hipModuleLaunchKernel(h->ljac_a, grid, 1, 1, LANE_BLOCK, 1, 1, 0, s, aa, nullptr));
-        void* ab[] = {&cnt, &s0, (void*)&param, (void*)&phi, &scr, &jac, &sN, &sI, (void*)&stride,
-                      &jN, &conp};
+        double* out = lines ? h->d_ljac_lines : jac;
+        long jN = lines || F ? 1 : ne;
+        long jS = lines ? s0 : 0;
+        long jI = lines ? cnt : sI;
+        void* ab[] = {&cnt, &s0, (void*)&param, (void*)&phi, &scr, &out, &sN, &sI, (void*)&stride,
+                      &jN, &jS, &jI, &conp};
         HIP_TRY(hipModuleLaunchKernel(h->ljac_b[layout], grid, 1, 1, LANE_BLOCK, 1, 1, 0, s, ab,
                                       nullptr));
+        if (lines) HIP_TRY(pjk_host::launch_entries_to_c(out, cnt, (int)ne, jac, s0, s));
     }
     return PYJAC_OK;
 }
@@ -1467,6 +1505,7 @@ int pyjac_hip_destroy(pyjac_handle* h) {
     if (h->lane_mod) (void)hipModuleUnload(h->lane_mod);
     for (hipModule_t m : h->ljac_mods) (void)hipModuleUnload(m);
     (void)hipFree(h->d_ljac_scr);
+    (void)hipFree(h->d_ljac_lines);
     delete h;
     return PYJAC_OK;
 }
@@ -1514,7 +1553,7 @@ int pyjac_hip_jacobian_dev_noalloc(pyjac_handle* h, size_t n, const double* para
     // point runs the block kernels the handle would run without it
     const int v1 = h->variant[1];
     const bool block = use_block(h, 1) ||
-                       (use_jac_module(h) && h->block_ok_jac && (v1 == 0 || v1 == 31));
+                       (use_jac_module(h) && h->block_ok_jac && (v1 == 0 || v1 == 31 || v1 == 32));
     if (block) {
         if (h->order == 'C') {
             HIP_TRY(pjk_host::launch_jacobian_block(h->m, h->d_args[1], h->lds[1],
@@ -1980,6 +2019,7 @@ int pyjac_hip_autotune(pyjac_handle* h, int kernel, size_t n, const double* d_pa
     if (!(kernel == 1 && h->bt.approx_sums)) cand.push_back(Cand{1, 0});
     if (kernel == 0 && h->lane_fn) cand.push_back(Cand{30, 0});
     if (kernel == 1 && jac_module_ready(h)) cand.push_back(Cand{31, 0});
+    if (kernel == 1 && ljac_lines_ready(h)) cand.push_back(Cand{32, 0});
     const int saved_variant = h->variant[kernel], saved_wps = h->wps[kernel];
     const int saved_wps_g = h->wps_g, saved_occ = h->occ[kernel];
     const long saved_chunk = h->fchunk_pref;
@@ -2039,8 +2079,8 @@ int pyjac_hip_autotune(pyjac_handle* h, int kernel, size_t n, const double* d_pa
     }
     h->occ[kernel] = best.occ;
     if (kernel == 1) h->fchunk_pref = best_chunk;
-    if (best.v == 31) {
-        h->variant[kernel] = 31;  // the Jacobian module, whatever the layout
+    if (best.v == 31 || best.v == 32) {
+        h->variant[kernel] = best.v;  // the Jacobian module, whatever the layout
     } else if (best.v == 30) {
         h->variant[kernel] = 0;  // the loaded module is the automatic choice
     } else if (best.v > 20) {
@@ -2050,7 +2090,7 @@ int pyjac_hip_autotune(pyjac_handle* h, int kernel, size_t n, const double* d_pa
     } else if (best.v > 10) {
         // with a module loaded the automatic choice would be the module
         const bool mod = kernel == 0 ? h->lane_fn != nullptr
-                                     : jac_module_ready(h);
+                                     : jac_module_ready(h) || ljac_lines_ready(h);
         h->variant[kernel] = mod ? best.v : 0;
         h->wps[kernel] = best.v - 10;
     } else {
@@ -2101,7 +2141,8 @@ int pyjac_hip_set_variant(pyjac_handle* h, int kernel, int variant) {
                   (variant > 10 && variant < 20 && wok && block) ||
                   (kernel == 1 && variant > 20 && variant < 30 && wok && h->glob_ok_jac) ||
                   (kernel == 0 && variant == 30 && h->lane_fn) ||
-                  (kernel == 1 && variant == 31 && jac_module_ready(h));
+                  (kernel == 1 && variant == 31 && jac_module_ready(h)) ||
+                  (kernel == 1 && variant == 32 && ljac_lines_ready(h));
         if (!ok)
             return fail(PYJAC_ERR_UNSUPPORTED, "variant %d cannot run this mechanism's %s", variant,
                         kernel == 0 ? "species rates" : "Jacobian");

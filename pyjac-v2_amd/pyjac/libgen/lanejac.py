@@ -10,7 +10,8 @@ lane evaluates one whole state in straight-line code in two kernels:
 
 * kernel A (``pyjac_ljac_a``) evaluates the species rates exactly as the
   specialised species-rate module (``lanegen``) does and, per reaction, every
-  value the Jacobian's sub-kernels need from it: kf, kr, the pressure
+  value the Jacobian's sub-kernels need from it: the PLOG and Chebyshev rate
+  constants (kernel B recomputes the Arrhenius ones), the pressure
   modification, the third-body/falloff derivative product, and the
   reaction's total contributions to the temperature and extra-variable
   columns (``create_jacobian.py:362-3230``); per species, the energy weights
@@ -25,8 +26,10 @@ lane evaluates one whole state in straight-line code in two kernels:
   bath-gas terms that every column of a row shares, with the temperature and
   extra-variable rows of the column (``dTdot_dnj``, ``dEdot_dnj``,
   ``:3266-3484``) summed over the column's rows in row order.  Every entry is
-  stored once, at the output position the runtime gives for the handle's
-  format and data order; in F order a wave's store is one whole line.
+  stored once, at its position in the handle's format and data order, with
+  the output strides the runtime gives: in F order a wave's store is one
+  whole line; in C order the runtime has the entries written as lines of a
+  chunk (at their C-order positions) and transposes them into C-order rows.
 
 The expressions are the checker's restatement of the reference's sub-kernels
 (``oracle/pyjac_oracle.cpp``), term for term with each table value replaced
@@ -46,7 +49,7 @@ import numpy as np
 from . import lanegen
 from .lanegen import lit, _powi, tables_hash
 
-LJAC_VERSION = 1
+LJAC_VERSION = 2
 KERNEL_A = 'pyjac_ljac_a'
 KERNEL_B = 'pyjac_ljac_b'
 SIG = 'pyjac_ljac_sig'
@@ -835,7 +838,7 @@ class LaneJacSource(lanegen.LaneRatesSource):
                                'sparsity pattern' % (row, col))
         self.entries.append((row, col))
         NN = self.NN
-        addr = 'o + (long)PJ_POS(%d, %d, %d, %d) * sI' % (
+        addr = 'o + (long)PJ_POS(%d, %d, %d, %d) * jI' % (
             self._crs[(row, col)], self._ccs[(row, col)], row * NN + col, col * NN + row)
         if self.nt_store:
             out.append('%sPJ_STO(%s, %s);' % (indent, addr, val))
@@ -941,10 +944,10 @@ class LaneJacSource(lanegen.LaneRatesSource):
                'PJ_FN void jb_state(long s, long sl, const double* __restrict__ param,',
                '                    const double* __restrict__ phi, double* __restrict__ scr,',
                '                    double* __restrict__ jac, double* __restrict__ bsh, int ln,',
-               '                    long sN, long sI, long jN, int conp) {',
+               '                    long sN, long sI, long jN, long jS, long jI, int conp) {',
                '#define BS(k) bsh[(k) * PJ_LANES + ln]',
                '    const double* ps = phi + s * sN;',
-               '    double* o = jac + s * jN;',
+               '    double* o = jac + (s - jS) * jN;',
                '    long ob = sl, op = s * sN;',
                '#define LD(v) scr[(long)(v) * PJ_SS + ob]',
                '    const double T = ps[0], x1 = ps[sI], pv = param[s];',
@@ -1260,7 +1263,7 @@ class LaneJacSource(lanegen.LaneRatesSource):
         for row in range(NN):
             for col in range(NN):
                 if (row, col) not in self.pattern:
-                    addr = 'o + (long)(L == 2 ? %d : %d) * sI' % (row * NN + col, col * NN + row)
+                    addr = 'o + (long)(L == 2 ? %d : %d) * jI' % (row * NN + col, col * NN + row)
                     out.append(('        PJ_STO(%s, 0.0);' if self.nt_store else '        *(%s) = 0.0;')
                                % addr)
         out.append('    }')
@@ -1325,12 +1328,12 @@ class LaneJacSource(lanegen.LaneRatesSource):
                         'extern "C" __global__ void __launch_bounds__(%d) %s%d(' % (BLOCK, KERNEL_B, L),
                         '    long n, long s0, const double* __restrict__ param, const double* __restrict__ phi,',
                         '    double* __restrict__ scr, double* __restrict__ jac, long sN, long sI,',
-                        '    long sS, long jN, int conp) {',
+                        '    long sS, long jN, long jS, long jI, int conp) {',
                         '    __shared__ double bsh[%d * %d];' % (Ns + self._nU, BLOCK),
                         '    const long sl = (long)blockIdx.x * %d + threadIdx.x;' % BLOCK,
                         '    if (sl < n) jb_state<%d>(s0 + sl, sl, param, phi, scr, jac, bsh, threadIdx.x,'
                         % L,
-                        '                              sN, sI, jN, conp);',
+                        '                              sN, sI, jN, jS, jI, conp);',
                         '}',
                         '#endif']
         else:
@@ -1344,10 +1347,31 @@ class LaneJacSource(lanegen.LaneRatesSource):
                     '    }',
                     '    for (long s = 0; s < n; ++s) {',
                     '        switch (layout) {',
-                    '            case 0: jb_state<0>(s, s, param, phi, scr, jac, bsh, 0, sN, sI, jN, conp); break;',
-                    '            case 1: jb_state<1>(s, s, param, phi, scr, jac, bsh, 0, sN, sI, jN, conp); break;',
-                    '            case 2: jb_state<2>(s, s, param, phi, scr, jac, bsh, 0, sN, sI, jN, conp); break;',
-                    '            default: jb_state<3>(s, s, param, phi, scr, jac, bsh, 0, sN, sI, jN, conp);',
+                    '            case 0: jb_state<0>(s, s, param, phi, scr, jac, bsh, 0, sN, sI, jN, 0, sI, conp); break;',
+                    '            case 1: jb_state<1>(s, s, param, phi, scr, jac, bsh, 0, sN, sI, jN, 0, sI, conp); break;',
+                    '            case 2: jb_state<2>(s, s, param, phi, scr, jac, bsh, 0, sN, sI, jN, 0, sI, conp); break;',
+                    '            default: jb_state<3>(s, s, param, phi, scr, jac, bsh, 0, sN, sI, jN, 0, sI, conp);',
+                    '        }',
+                    '    }',
+                    '}',
+                    '// the store kernel of a layout with explicit output strides (entry p of',
+                    '// state s at jac[(s - jS) * jN + p * jI]), as the runtime launches it to',
+                    '// write C-order positions as lines of a chunk',
+                    'extern "C" void pyjac_ljac_host_strided(long n, const double* param,',
+                    '    const double* phi, double* scr, double* jac, int layout, long sN, long sI,',
+                    '    long jN, long jS, long jI, int conp) {',
+                    '    double bsh[%d];' % (Ns + self._nU),
+                    '    pj_ss = n;',
+                    '    for (long s = 0; s < n; ++s) {',
+                    '        if (conp) ja_state<1>(s, s, param, phi, scr, bsh, 0, sN, sI);',
+                    '        else ja_state<0>(s, s, param, phi, scr, bsh, 0, sN, sI);',
+                    '    }',
+                    '    for (long s = 0; s < n; ++s) {',
+                    '        switch (layout) {',
+                    '            case 0: jb_state<0>(s, s, param, phi, scr, jac, bsh, 0, sN, sI, jN, jS, jI, conp); break;',
+                    '            case 1: jb_state<1>(s, s, param, phi, scr, jac, bsh, 0, sN, sI, jN, jS, jI, conp); break;',
+                    '            case 2: jb_state<2>(s, s, param, phi, scr, jac, bsh, 0, sN, sI, jN, jS, jI, conp); break;',
+                    '            default: jb_state<3>(s, s, param, phi, scr, jac, bsh, 0, sN, sI, jN, jS, jI, conp);',
                     '        }',
                     '    }',
                     '}']

@@ -34,10 +34,15 @@ def _compile(src, d, name):
     res = subprocess.run([gxx, '-O1', '-ffp-contract=off', '-shared', '-fPIC', '-o', so, cpp],
                          stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
     assert res.returncode == 0, res.stdout[-3000:]
-    fn = ctypes.CDLL(so).pyjac_ljac_host
+    lib = ctypes.CDLL(so)
+    fn = lib.pyjac_ljac_host
     fn.argtypes = [ctypes.c_long, D, D, D, D, ctypes.c_int, ctypes.c_long, ctypes.c_long,
                    ctypes.c_long, ctypes.c_int]
+    _LIBS[id(fn)] = (fn, lib)
     return fn
+
+
+_LIBS = {}
 
 
 def _host(mech, name, tmp_path_factory):
@@ -45,6 +50,13 @@ def _host(mech, name, tmp_path_factory):
         d = tmp_path_factory.mktemp('ljac_' + name)
         _BUILT[name] = _compile(lanejac.generate(mech, 'host'), d, name)
     return _BUILT[name]
+
+
+def _strided(fn):
+    """The host build's store kernel with explicit output strides."""
+    g = _LIBS[id(fn)][1].pyjac_ljac_host_strided
+    g.argtypes = [ctypes.c_long, D, D, D, D, ctypes.c_int] + [ctypes.c_long] * 5 + [ctypes.c_int]
+    return g
 
 
 def _run(fn, mech, conp, param, phi, order, fmt):
@@ -98,6 +110,33 @@ def test_host_build_guards(mechs, tmp_path_factory, name):
         fin = np.isfinite(ref).all(axis=1)
         assert (np.isfinite(got).all(axis=1) == fin).all()
         assert_jacobian_close(got[fin], ref[fin], '%s guards conp=%s' % (name, conp))
+
+
+@pytest.mark.parametrize('name', ['h2o2', 'gri30', 'test'])
+def test_c_positions_as_lines(mechs, tmp_path_factory, name):
+    """The runtime's C-order path through lines: the C-layout store kernel
+    writing its C-order positions as lines of the states (entry stride = the
+    number of states, state offset 0), transposed, is its C-order output bit
+    for bit, sparse and full, CONP and CONV."""
+    mech = mechs(name)
+    g = _strided(_host(mech, name, tmp_path_factory))
+    NN = mech.NN
+    for conp in (True, False):
+        param, phi = sample_states(mech, 150, seed=9, T_range=(900.0, 2800.0), alpha=0.3,
+                                   conp=conp)
+        n = len(param)
+        phi = np.ascontiguousarray(phi)
+        for layout, ne in ((0, mech.nnz), (2, NN * NN)):
+            outs = []
+            for jN, jI in ((ne, 1), (1, n)):
+                scr = np.zeros(lanejac.LaneJacSource(mech).nscr * n)
+                out = np.full(n * ne, np.nan)
+                g(n, param.ctypes.data_as(D), phi.ctypes.data_as(D), scr.ctypes.data_as(D),
+                  out.ctypes.data_as(D), layout, NN, 1, jN, 0, jI, int(conp))
+                outs.append(out)
+            direct, lines = outs
+            assert not np.isnan(lines).any()
+            assert np.array_equal(direct, lines.reshape(ne, n).T.reshape(-1), equal_nan=True)
 
 
 def test_planted_mistake_fails(mechs, tmp_path_factory):

@@ -319,7 +319,7 @@ size_t pyjac_hip_required_memory(const pyjac_handle *h);
    by pyjac_hip_load_module (the automatic choice once loaded), 31
    (Jacobian) the mechanism-specialised Jacobian module storing the output
    directly (the automatic choice once loaded in F order), 32 (Jacobian, C
-   order) the module's C-layout store kernel writing lines of a chunk
+   order) the module's F-layout store kernel writing lines of a chunk
    (entry stride = the chunk's states) that a tiled transpose writes out as
    C-order rows (the automatic choice once loaded in C order).
    PYJAC_ERR_UNSUPPORTED when this mechanism cannot run the variant (its

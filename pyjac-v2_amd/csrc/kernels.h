@@ -131,11 +131,13 @@ hipError_t launch_jacobian_block(const DevMech& m, const BlockArgs* args, const 
                                  long chunk_states, hipStream_t stream,
                                  const FPipeline* pipe = nullptr, int occ = 0);
 
-// count states' entries, F-order lines of a chunk (entry e of state i at
-// src[e * count + i]), into C-order output rows s0 .. s0 + count - 1 of nout
-// values each (the specialised Jacobian module's C-order path)
+// count states' entries, F-order lines of a chunk (the entry at F-layout
+// position p of state i at src[p * count + i]), into C-order output rows
+// s0 .. s0 + count - 1 of nout values each, C-order entry e from position
+// fpos[e] (device array of nout; the specialised Jacobian module's C-order
+// path)
 hipError_t launch_entries_to_c(const double* src, long count, int nout, double* jac, long s0,
-                               hipStream_t stream);
+                               const int* fpos, hipStream_t stream);
 
 // one workgroup of 64 * wps threads per state, Jacobian gathered in device
 // memory, chunk_states states (a multiple of 64) per launch: chunk_buf holds

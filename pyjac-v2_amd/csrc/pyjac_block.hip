@@ -1717,15 +1717,17 @@ entries_to_f2_small_kernel(const double* __restrict__ src, long count, int nout,
 }
 
 // The other direction, for the specialised Jacobian module in C order: its
-// store kernel writes a chunk's entries as F-order lines (entry e of chunk
-// state i at src[e * count + i], whole lines per wave) and this kernel writes
-// them out as C-order rows (state s0 + i at dst[(s0 + i) * nout]).  32 x 32
-// tiles through LDS: a wave reads two 256-byte runs of one entry's states and
-// writes two 256-byte runs of one state's entries.  Both arrays are touched
-// once: non-temporal loads and stores.
+// F-layout store kernel writes a chunk's entries as F-order lines (the entry
+// at F-layout position p of chunk state i at src[p * count + i], whole lines
+// per wave, non-temporal stores) and this kernel writes them out as C-order
+// rows (state s0 + i at dst[(s0 + i) * nout]), C-order entry e read from the
+// line of its F-layout position fpos[e].  32 x 32 tiles through LDS: a wave
+// reads two 256-byte runs of one entry's states and writes two 256-byte runs
+// of one state's entries.  Both arrays are touched once: non-temporal loads
+// and stores.
 __global__ void __launch_bounds__(256)
 entries_to_c_kernel(const double* __restrict__ src, long count, int nout,
-                    double* __restrict__ dst, long s0) {
+                    double* __restrict__ dst, long s0, const int* __restrict__ fpos) {
     __shared__ double tile[32][33];
     const long i0 = (long)blockIdx.x * 32;
     const int e0 = blockIdx.y * 32;
@@ -1733,7 +1735,8 @@ entries_to_c_kernel(const double* __restrict__ src, long count, int nout,
     for (int r = ty; r < 32; r += 8) {
         const int e = e0 + r;
         const long i = i0 + tx;
-        if (e < nout && i < count) tile[r][tx] = __builtin_nontemporal_load(src + (size_t)e * count + i);
+        if (e < nout && i < count)
+            tile[r][tx] = __builtin_nontemporal_load(src + (size_t)fpos[e] * count + i);
     }
     __syncthreads();
     for (int r = ty; r < 32; r += 8) {
@@ -2066,11 +2069,11 @@ static hipError_t launch_jacobian_block_chunk(const DevMech& m, const BlockArgs*
 }
 
 hipError_t launch_entries_to_c(const double* src, long count, int nout, double* jac, long s0,
-                               hipStream_t stream) {
+                               const int* fpos, hipStream_t stream) {
     if (count <= 0 || nout <= 0) return hipSuccess;
     dim3 grid((unsigned)((count + 31) / 32), (unsigned)((nout + 31) / 32));
     hipLaunchKernelGGL(pjb::entries_to_c_kernel, grid, dim3(256), 0, stream, src, count, nout,
-                       jac, s0);
+                       jac, s0, fpos);
     return hipGetLastError();
 }
 

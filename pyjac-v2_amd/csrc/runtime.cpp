@@ -133,9 +133,13 @@ struct pyjac_handle {
     double* d_ljac_scr = nullptr;
     size_t ljac_scr_doubles = 0;
     // C order through F-order lines (variant 32): one chunk's entries, which
-    // a tiled transpose writes out as C-order rows
+    // a tiled transpose writes out as C-order rows, and the F-layout position
+    // of each C-order entry (host copy from the tables, device copy made on
+    // first use)
     double* d_ljac_lines = nullptr;
     size_t ljac_lines_doubles = 0;
+    std::vector<int> ljac_fpos;
+    int* d_ljac_fpos = nullptr;
     // the stored sparsity pattern, row * NN + col, whatever the format
     std::vector<int> pattern;
 };
@@ -222,12 +226,14 @@ bool jac_module_ready(const pyjac_handle* h) {
     return h->ljac_a && h->ljac_b[ljac_layout(h)] && h->jac_type == PYJAC_JAC_EXACT;
 }
 
-// C order through lines (variant 32): the handle's own store kernel, whose
-// entry positions are those of C order, writes a chunk's entries with the
-// strides of F-order lines (entry stride = the chunk's states), and a tiled
-// transpose writes the lines out as C-order rows
+// C order through lines (variant 32): the store kernel of the handle's
+// format in F layout writes a chunk's entries as lines (entry stride = the
+// chunk's states, non-temporal stores), and a tiled transpose writes them out
+// as C-order rows, each C-order entry read from the line of its F-layout
+// position
 bool ljac_lines_ready(const pyjac_handle* h) {
-    return h->order == 'C' && jac_module_ready(h);
+    return h->order == 'C' && h->ljac_a && h->ljac_b[ljac_layout(h) | 1] &&
+           h->jac_type == PYJAC_JAC_EXACT && !h->ljac_fpos.empty();
 }
 
 // the module runs through lines and the transpose: asked for (variant 32),
@@ -535,10 +541,10 @@ int run_fd_jac(pyjac_handle* h, long n, const double* param, const double* phi, 
 // the specialised Jacobian module: per chunk of at most `ljac_stride`
 // states, kernel A writes the rates' per-reaction and per-species values to
 // the scratch array ([value][state in chunk]) and kernel B stores every entry
-// at its C- or F-order position p of the handle's layout (jac + (s - jS) * jN
-// + p * jI), or, through lines, at its C-order position in lines of the
-// chunk (jN = 1, jS = s0, jI = the chunk's states) that a tiled transpose
-// writes out as C-order rows; one state per lane, 64 per workgroup
+// at its position p of the handle's layout (jac + (s - jS) * jN + p * jI),
+// or, through lines, at its F-layout position in lines of the chunk (jN = 1,
+// jS = s0, jI = the chunk's states) that a tiled transpose writes out as
+// C-order rows; one state per lane, 64 per workgroup
 int grow_buffer(double** buf, size_t* have, size_t need) {
     if (need <= *have) return PYJAC_OK;
     if (*buf) (void)hipFree(*buf);
@@ -559,8 +565,13 @@ int launch_jac_module(pyjac_handle* h, long n, const double* param, const double
     if (!rc && lines)
         rc = grow_buffer(&h->d_ljac_lines, &h->ljac_lines_doubles, (size_t)ne * std::min(n, stride));
     if (rc) return rc;
+    if (lines && !h->d_ljac_fpos) {
+        HIP_TRY(hipMalloc(&h->d_ljac_fpos, h->ljac_fpos.size() * sizeof(int)));
+        HIP_TRY(hipMemcpy(h->d_ljac_fpos, h->ljac_fpos.data(), h->ljac_fpos.size() * sizeof(int),
+                          hipMemcpyHostToDevice));
+    }
     const bool F = h->order == 'F';
-    const int layout = ljac_layout(h);
+    const int layout = lines ? (ljac_layout(h) | 1) : ljac_layout(h);
     long sN = F ? 1 : h->m.NN, sI = F ? n : 1;
     int conp = h->conp;
     double* scr = h->d_ljac_scr;
@@ -577,7 +588,8 @@ int launch_jac_module(pyjac_handle* h, long n, const double* param, const double
                       &jN, &jS, &jI, &conp};
         HIP_TRY(hipModuleLaunchKernel(h->ljac_b[layout], grid, 1, 1, LANE_BLOCK, 1, 1, 0, s, ab,
                                       nullptr));
-        if (lines) HIP_TRY(pjk_host::launch_entries_to_c(out, cnt, (int)ne, jac, s0, s));
+        if (lines)
+            HIP_TRY(pjk_host::launch_entries_to_c(out, cnt, (int)ne, jac, s0, h->d_ljac_fpos, s));
     }
     return PYJAC_OK;
 }
@@ -1061,6 +1073,25 @@ int pyjac_hip_create(const pyjac_mech_tables* mt, int conp, char order, int jac_
     for (int row = 0; row < NN; ++row)
         for (int q = mt->crs_row_ptr[row]; q < mt->crs_row_ptr[row + 1]; ++q)
             h->pattern.push_back(row * NN + mt->crs_col_ind[q]);
+    // F-layout position of each C-order entry: column-major within the state
+    // (full), the CCS index of the CRS entry (sparse)
+    if (order == 'C') {
+        if (jac_format == PYJAC_JAC_FULL) {
+            for (int row = 0; row < NN; ++row)
+                for (int col = 0; col < NN; ++col) h->ljac_fpos.push_back(col * NN + row);
+        } else {
+            std::vector<int> ccs((size_t)NN * NN, -1);
+            for (int col = 0; col < NN; ++col)
+                for (int q = mt->ccs_col_ptr[col]; q < mt->ccs_col_ptr[col + 1]; ++q)
+                    ccs[(size_t)mt->ccs_row_ind[q] * NN + col] = q;
+            for (int e : h->pattern) h->ljac_fpos.push_back(ccs[e]);
+            for (int q : h->ljac_fpos)
+                if (q < 0 || q >= (int)h->pattern.size()) {
+                    h->ljac_fpos.clear();  // inconsistent CRS / CCS tables: no lines path
+                    break;
+                }
+        }
+    }
 
     // derived tables
     std::vector<int> thd_only, fall_order, jac_pos;
@@ -1506,6 +1537,7 @@ int pyjac_hip_destroy(pyjac_handle* h) {
     for (hipModule_t m : h->ljac_mods) (void)hipModuleUnload(m);
     (void)hipFree(h->d_ljac_scr);
     (void)hipFree(h->d_ljac_lines);
+    (void)hipFree(h->d_ljac_fpos);
     delete h;
     return PYJAC_OK;
 }

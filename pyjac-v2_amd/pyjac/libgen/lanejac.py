@@ -236,8 +236,9 @@ def _nu(nr, np_):
 class LaneJacSource(lanegen.LaneRatesSource):
     """The generated Jacobian source for one mechanism's tables."""
 
-    def __init__(self, mt, nt_store=True, u_lds=True):
+    def __init__(self, mt, nt_store=True, u_lds=True, db_depth=3):
         super(LaneJacSource, self).__init__(mt)
+        self.db_depth = db_depth
         # the entries are stored non-temporally (they are not read again, and
         # the state that the columns reload stays in L2); the bath-gas terms
         # of the rows live in LDS beside the Gibbs terms instead of registers
@@ -724,17 +725,21 @@ class LaneJacSource(lanegen.LaneRatesSource):
             out.append(self._store('DB', k, 'dgibbs(Tc, Tinv, %s)' % self._coeffs(k, 6, 'dq')))
         out.append('    double %s;' % ', '.join('w%d = 0.0' % k for k in range(Ns)))
         out.append('    double %s;' % ', '.join('jt%d = 0.0' % k for k in range(Ns - 1)))
-        # reaction r + 1's scratch loads (through a fresh opaque offset, so that
-        # loads of one value by several reactions are not merged into one
-        # long-lived register) are issued before reaction r's block and pinned
-        # after it
-        loads, pins = self._db_loads(0)
-        out += loads + pins
+        # reaction r + D's scratch loads (through a fresh opaque offset, so
+        # that loads of one value by several reactions are not merged into one
+        # long-lived register) are issued before reaction r's block, and
+        # reaction r + 1's are pinned after it: each reaction's loads are in
+        # flight over D reaction blocks, and waiting for them does not wait
+        # for the stores issued after them
+        D = self.db_depth
+        dl = [self._db_loads(r) for r in range(self.Nr)] + [([], [])] * D
+        for r in range(D):
+            out += dl[r][0]
+        out += dl[0][1]
         for r in range(self.Nr):
-            nxt = self._db_loads(r + 1) if r + 1 < self.Nr else ([], [])
-            out += nxt[0]
+            out += dl[r + D][0]
             out += self._reaction(r)
-            out += nxt[1]
+            out += dl[r + 1][1]
         # energies, heat capacities, temperature rate; the temperature column's
         # corner entries; the per-species terms of the other rows
         out += ['    double Tq = Tc;', '    PJ_OPAQUE(Tq);']

@@ -113,30 +113,38 @@ def test_host_build_guards(mechs, tmp_path_factory, name):
 
 
 @pytest.mark.parametrize('name', ['h2o2', 'gri30', 'test'])
-def test_c_positions_as_lines(mechs, tmp_path_factory, name):
-    """The runtime's C-order path through lines: the C-layout store kernel
-    writing its C-order positions as lines of the states (entry stride = the
-    number of states, state offset 0), transposed, is its C-order output bit
-    for bit, sparse and full, CONP and CONV."""
+def test_c_order_through_f_lines(mechs, tmp_path_factory, name):
+    """The runtime's C-order path through lines: the F-layout store kernel
+    writing lines of the states (entry stride = the number of states, state
+    offset 0), each C-order entry then read from the line of its F-layout
+    position (the CCS index of the CRS entry; column-major for full), is the
+    C-layout kernel's C-order output bit for bit, sparse and full, CONP and
+    CONV."""
     mech = mechs(name)
     g = _strided(_host(mech, name, tmp_path_factory))
     NN = mech.NN
+    rc, cc = mech.sparse_rows_cols('C')
+    rf, cf = mech.sparse_rows_cols('F')
+    ccs = {(int(a), int(b)): q for q, (a, b) in enumerate(zip(rf, cf))}
+    fpos_sparse = np.array([ccs[(int(a), int(b))] for a, b in zip(rc, cc)])
+    fpos_full = np.array([col * NN + row for row in range(NN) for col in range(NN)])
     for conp in (True, False):
         param, phi = sample_states(mech, 150, seed=9, T_range=(900.0, 2800.0), alpha=0.3,
                                    conp=conp)
         n = len(param)
         phi = np.ascontiguousarray(phi)
-        for layout, ne in ((0, mech.nnz), (2, NN * NN)):
+        for layout, ne, fpos in ((0, mech.nnz, fpos_sparse), (2, NN * NN, fpos_full)):
             outs = []
-            for jN, jI in ((ne, 1), (1, n)):
+            for L, jN, jI in ((layout, ne, 1), (layout | 1, 1, n)):
                 scr = np.zeros(lanejac.LaneJacSource(mech).nscr * n)
                 out = np.full(n * ne, np.nan)
                 g(n, param.ctypes.data_as(D), phi.ctypes.data_as(D), scr.ctypes.data_as(D),
-                  out.ctypes.data_as(D), layout, NN, 1, jN, 0, jI, int(conp))
+                  out.ctypes.data_as(D), L, NN, 1, jN, 0, jI, int(conp))
                 outs.append(out)
             direct, lines = outs
             assert not np.isnan(lines).any()
-            assert np.array_equal(direct, lines.reshape(ne, n).T.reshape(-1), equal_nan=True)
+            back = lines.reshape(ne, n)[fpos].T.reshape(-1)
+            assert np.array_equal(direct, back, equal_nan=True)
 
 
 def test_planted_mistake_fails(mechs, tmp_path_factory):

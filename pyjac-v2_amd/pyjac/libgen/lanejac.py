@@ -69,6 +69,7 @@ TARGET_HIP = r'''// generated by pyjac.libgen.lanejac -- mechanism-specialised a
 #define PJ_SS %d
 #define PJ_LANES 64
 #define PJ_NTST(p, v) __builtin_nontemporal_store((double)(v), (p))
+#define PJ_NTLD(p) __builtin_nontemporal_load(p)
 '''
 
 TARGET_HOST = r'''// generated by pyjac.libgen.lanejac -- host build of the specialised Jacobian
@@ -81,6 +82,7 @@ static long pj_ss = 1;
 #define PJ_SS pj_ss
 #define PJ_LANES 1
 #define PJ_NTST(p, v) (*(p) = (v))
+#define PJ_NTLD(p) (*(p))
 static inline double __dadd_rn(double a, double b) { return a + b; }
 static inline double __dmul_rn(double a, double b) { return a * b; }
 '''
@@ -236,9 +238,10 @@ def _nu(nr, np_):
 class LaneJacSource(lanegen.LaneRatesSource):
     """The generated Jacobian source for one mechanism's tables."""
 
-    def __init__(self, mt, nt_store=True, u_lds=True, db_depth=3):
+    def __init__(self, mt, nt_store=True, u_lds=True, db_depth=3, nt_loads=True):
         super(LaneJacSource, self).__init__(mt)
         self.db_depth = db_depth
+        self.nt_loads = nt_loads
         # the entries are stored non-temporally (they are not read again, and
         # the state that the columns reload stays in L2); the bath-gas terms
         # of the rows live in LDS beside the Gibbs terms instead of registers
@@ -927,6 +930,7 @@ class LaneJacSource(lanegen.LaneRatesSource):
         self.entries = []
         self._crs, self._ccs = self._positions()
         LD = lambda v: 'LD(%d)' % v
+        LDN = lambda v: ('LDN(%d)' if self.nt_loads else 'LD(%d)') % v
         KF = lambda r: 'kf%d' % r
         KR = lambda r: 'kr%d' % r
         CI = lambda r: LD(self.sv[('CI', r)])
@@ -955,6 +959,9 @@ class LaneJacSource(lanegen.LaneRatesSource):
                '    double* o = jac + (s - jS) * jN;',
                '    long ob = sl, op = s * sN;',
                '#define LD(v) scr[(long)(v) * PJ_SS + ob]',
+               '// scratch values read once: non-temporal loads, which leave the L2 to',
+               '// the state and the values that several columns read',
+               '#define LDN(v) PJ_NTLD(&scr[(long)(v) * PJ_SS + ob])',
                '    const double T = ps[0], x1 = ps[sI], pv = param[s];',
                '    const double P = conp ? pv : x1;',
                '    const double V = conp ? x1 : pv;',
@@ -982,7 +989,7 @@ class LaneJacSource(lanegen.LaneRatesSource):
             out.append('    BS(%d) = gibbs(Tc, logT, Tinv, %s);' % (k, self._coeffs(k, 7)))
         # the energy weights of the temperature row, read once
         for k in range(Ns - 1):
-            out.append('    const double ew%d = LD(%d);' % (k, self.sv[('EW', k)]))
+            out.append('    const double ew%d = %s;' % (k, LDN(self.sv[('EW', k)])))
 
         # ---- the temperature column (kernel A's) and the extra-variable
         # column.  Scratch values are loaded in batches, each batch pinned by
@@ -1001,9 +1008,9 @@ class LaneJacSource(lanegen.LaneRatesSource):
             return body
 
         out.append('    {  // column 0')
-        col0 = [(i + 2, 'jt_%d' % i, LD(self.sv[('JT', i)])) for i in range(Ns - 1)
+        col0 = [(i + 2, 'jt_%d' % i, LDN(self.sv[('JT', i)])) for i in range(Ns - 1)
                 if (i + 2, 0) in self.pattern]
-        col0 += [(0, 'jt_T', LD(self.sv[('JTT', 0)])), (1, 'jt_E', LD(self.sv[('JET', 0)]))]
+        col0 += [(0, 'jt_T', LDN(self.sv[('JTT', 0)])), (1, 'jt_E', LDN(self.sv[('JET', 0)]))]
         rowof = {nm: row for row, nm, _ in col0}
 
         def store0(batch):
@@ -1030,9 +1037,9 @@ class LaneJacSource(lanegen.LaneRatesSource):
                             body.append('        e_%d = e_%d + %s * %s;' % (k, k, lit(_nu(nr, np_)), nm))
                 return body
             # each value once, its rows' sums in reaction order
-            out += batched([('ee_%d' % r, LD(self.sv[('EE', r)])) for r in rxn], add, '        ')
+            out += batched([('ee_%d' % r, LDN(self.sv[('EE', r)])) for r in rxn], add, '        ')
             if mode:
-                out += batched([('wd_%d' % i, LD(self.sv[('WD', i)])) for i in range(Ns - 1)],
+                out += batched([('wd_%d' % i, LDN(self.sv[('WD', i)])) for i in range(Ns - 1)],
                                lambda b: [], '        ')
             out.append('        double specsum = 0.0, esum = 0.0;')
             for i in range(Ns - 1):
@@ -1190,7 +1197,7 @@ class LaneJacSource(lanegen.LaneRatesSource):
             for r, dci in thd_cols.get(j, []):
                 loads.append('    double g%d_%d = %s;' % (j, r, G(r)))
                 pins.append('g%d_%d' % (j, r))
-            loads.append('    double dcp%d = LD(%d);' % (j, self.sv[('DCP', j)]))
+            loads.append('    double dcp%d = %s;' % (j, LDN(self.sv[('DCP', j)])))
             pins.append('dcp%d' % j)
             comp = ['    // column %d' % col]
             for k in rows:
@@ -1273,6 +1280,7 @@ class LaneJacSource(lanegen.LaneRatesSource):
                                % addr)
         out.append('    }')
         out.append('#undef LD')
+        out.append('#undef LDN')
         out.append('#undef BS')
         out.append('}')
         out.append('#undef PJ_POS')

@@ -297,10 +297,17 @@ class LaneRatesSource(object):
     # ---------------------------------------------------------------- parts
     def _species(self):
         A, Ns = self.A, self.Ns
-        out = ['    // concentrations; the bath gas by difference (rate_subs.py:597-706)',
+        out = ['    // concentrations; the bath gas by difference (rate_subs.py:597-706).',
+               '    // The state\'s moles are loaded together and pinned by empty asm',
+               '    // statements before their first use, so that the loads are in flight',
+               '    // together instead of one after another',
                '    double nsum = 0.0;']
         for k in range(Ns - 1):
-            out.append('    const double n%d = fmax(SMALL, ps[%d * sI]);' % (k, k + 2))
+            out.append('    double n%d = ps[%d * sI];' % (k, k + 2))
+        for k in range(Ns - 1):
+            out.append('    PJ_OPAQUE(n%d);' % k)
+        for k in range(Ns - 1):
+            out.append('    n%d = fmax(SMALL, n%d);' % (k, k))
             out.append('    nsum = nsum + n%d;' % k)
             out.append('    const double c%d = n%d * V_inv;' % (k, k))
         out.append('    const double c%d = __dadd_rn(P / (RU * Tc), '

@@ -970,10 +970,13 @@ class LaneJacSource(lanegen.LaneRatesSource):
                '    const double Vinv = 1.0 / V;',
                '    const double dTdt = LD(%d), cptot = LD(%d);'
                % (self.sv[('DTDT', 0)], self.sv[('CPTOT', 0)])]
-        # the bath gas by difference, exactly as kernel A computes it
+        # the bath gas by difference, exactly as kernel A computes it; the
+        # state's moles loaded together and pinned before their first use
         out.append('    double nsum = 0.0;')
+        out += ['    double nb%d = ps[%d * sI];' % (k, k + 2) for k in range(Ns - 1)]
+        out += ['    PJ_OPAQUE(nb%d);' % k for k in range(Ns - 1)]
         for k in range(Ns - 1):
-            out.append('    nsum = nsum + fmax(SMALL, ps[%d * sI]);' % (k + 2))
+            out.append('    nsum = nsum + fmax(SMALL, nb%d);' % k)
         out.append('    const double cg%d = fmax(SMALL, __dadd_rn(P / (RU * Tc), '
                    '__dmul_rn(-1.0 * fmax(SMALL, nsum), V_inv)));' % ns)
         # the Gibbs terms of the reverse rate constants, as kernel A has them
@@ -989,7 +992,8 @@ class LaneJacSource(lanegen.LaneRatesSource):
             out.append('    BS(%d) = gibbs(Tc, logT, Tinv, %s);' % (k, self._coeffs(k, 7)))
         # the energy weights of the temperature row, read once
         for k in range(Ns - 1):
-            out.append('    const double ew%d = %s;' % (k, LDN(self.sv[('EW', k)])))
+            out.append('    double ew%d = %s;' % (k, LDN(self.sv[('EW', k)])))
+        out += ['    PJ_OPAQUE(ew%d);' % k for k in range(Ns - 1)]
 
         # ---- the temperature column (kernel A's) and the extra-variable
         # column.  Scratch values are loaded in batches, each batch pinned by

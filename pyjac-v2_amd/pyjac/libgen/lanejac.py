@@ -1000,13 +1000,19 @@ class LaneJacSource(lanegen.LaneRatesSource):
         # an empty asm statement before its first use, so that the loads of a
         # batch are in flight together
         def batched(names_vals, consume, indent='    ', size=16):
-            """Loads `name = value` in batches of `size`, pinned, each batch
-            followed by consume(batch) statements."""
-            body = []
-            for b0 in range(0, len(names_vals), size):
-                batch = names_vals[b0:b0 + size]
-                body.append('%sPJ_OPAQUE(ob);' % indent)
-                body += ['%sdouble %s = %s;' % (indent, nm, val) for nm, val in batch]
+            """Loads `name = value` in batches of `size`, each batch followed
+            by consume(batch) statements; batch k + 1's loads are issued
+            before batch k is pinned and consumed, so that one batch is in
+            flight while the previous one is used."""
+            batches = [names_vals[b0:b0 + size] for b0 in range(0, len(names_vals), size)]
+
+            def issue(batch):
+                return (['%sPJ_OPAQUE(ob);' % indent] +
+                        ['%sdouble %s = %s;' % (indent, nm, val) for nm, val in batch])
+            body = issue(batches[0]) if batches else []
+            for k, batch in enumerate(batches):
+                if k + 1 < len(batches):
+                    body += issue(batches[k + 1])
                 body += ['%sPJ_OPAQUE(%s);' % (indent, nm) for nm, _ in batch]
                 body += consume(batch)
             return body

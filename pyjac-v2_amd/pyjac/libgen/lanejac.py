@@ -904,15 +904,45 @@ class LaneJacSource(lanegen.LaneRatesSource):
         need_kr = need_kr and self.rev[r]
         out = []
         if need_kf or not need_kr:
-            out.append('%sconst double kf%d = %s;' % (indent, r, kf))
+            decl, e = self._opaque_literals(kf, 'kq%d_' % r, indent)
+            out += decl
+            out.append('%sconst double kf%d = %s;' % (indent, r, e))
         if need_kr:
             if not need_kf and exp_form:
                 assert kf.startswith('ex(') and kf.endswith(')')
                 kr = self._kr_product(r, None, kf[3:-1])
             else:
                 kr = self._kr_product(r, 'kf%d' % r if need_kf else '(%s)' % kf)
-            out.append('%sconst double kr%d = %s;' % (indent, r, kr))
+            decl, e = self._opaque_literals(kr, 'rq%d_' % r, indent)
+            out += decl
+            out.append('%sconst double kr%d = %s;' % (indent, r, e))
         return out
+
+    # a literal of the generated code (lit()): a decimal point or an exponent,
+    # negative ones in parentheses
+    _LITERAL = re.compile(r'(?<![\w.])(\(-(?:\d+\.\d*(?:e[+-]?\d+)?|\d+e[+-]?\d+)\)'
+                          r'|(?:\d+\.\d*(?:e[+-]?\d+)?|\d+e[+-]?\d+))')
+    # literals that are inline constants of the instruction set
+    _INLINE = {0.0, 0.5, 1.0, 2.0, 4.0, -0.5, -1.0, -2.0, -4.0}
+
+    def _opaque_literals(self, expr, prefix, indent):
+        """The literals of `expr` (other than inline constants) as local
+        copies, each pinned in scalar registers by an empty asm statement:
+        without it the compiler keeps a constant that several columns use
+        (a reaction's log A, 1 / P_sum) in a vector register from the first
+        column to the last, and spills it.  (declarations, expr)"""
+        decl, names = [], {}
+
+        def sub(m):
+            tok = m.group(1)
+            if float(tok.strip('()')) in self._INLINE:
+                return tok
+            if tok not in names:
+                nm = '%s%d' % (prefix, len(names))
+                names[tok] = nm
+                decl.append('%sdouble %s = %s; PJ_OPQ(%s);' % (indent, nm, tok, nm))
+            return names[tok]
+        return decl, self._LITERAL.sub(sub, expr)
 
     def _cg_defs(self, species, indent='        '):
         """Guarded concentrations of `species` recomputed from the state (the

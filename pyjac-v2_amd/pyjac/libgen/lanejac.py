@@ -238,8 +238,10 @@ def _nu(nr, np_):
 class LaneJacSource(lanegen.LaneRatesSource):
     """The generated Jacobian source for one mechanism's tables."""
 
-    def __init__(self, mt, nt_store=True, u_lds=True, db_depth=3, nt_loads=True):
+    def __init__(self, mt, nt_store=True, u_lds=True, db_depth=3, nt_loads=True,
+                 early_loads=True):
         super(LaneJacSource, self).__init__(mt)
+        self.early_loads = early_loads
         self.db_depth = db_depth
         self.nt_loads = nt_loads
         # the entries are stored non-temporally (they are not read again, and
@@ -1197,9 +1199,10 @@ class LaneJacSource(lanegen.LaneRatesSource):
         # each column in three parts: its loads (state and scratch values,
         # issued together and then pinned by an empty asm statement, so that
         # the column waits once), its arithmetic, and its stores; column j+1's
-        # loads are issued before column j's stores and pinned after them, so
-        # that the wait for them does not include those stores (one counter
-        # covers loads and stores, in issue order)
+        # loads are issued before column j's arithmetic (early_loads; else
+        # before its stores) and pinned after its stores, so that they are in
+        # flight during the arithmetic and the wait for them does not include
+        # those stores (one counter covers loads and stores, in issue order)
         parts = []
         for j in range(Ns - 1):
             col = j + 2
@@ -1298,9 +1301,11 @@ class LaneJacSource(lanegen.LaneRatesSource):
         if parts:
             out += parts[0][0] + parts[0][1]
         for j, (loads, pins, comp, stores) in enumerate(parts):
-            out += comp
             nxt = parts[j + 1] if j + 1 < len(parts) else None
-            if nxt:
+            if nxt and self.early_loads:
+                out += nxt[0]
+            out += comp
+            if nxt and not self.early_loads:
                 out += nxt[0]
             out += stores
             if nxt:
